@@ -1,0 +1,43 @@
+// Host-side launchers of the gfx950 FFT kernels (implemented in
+// fft_kernels.hip). All pointers are device-accessible; every launcher only
+// enqueues work on `stream` and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_plan.h"
+
+namespace fftlib {
+
+// Scaling variants of the complex inverse transform.
+enum C2CMode : int {
+   C2C_FORWARD          = 0,
+   C2C_INVERSE_PRESCALE = 1,  // batched IFFT: conj, x (1.0f/N) up front, conj on store
+   C2C_INVERSE_POSTDIV  = 2,  // single-channel IFFT: conj up front, / (float)N and conj on store
+};
+
+// Batched complex FFT of st.n points per channel, channels contiguous.
+hipError_t launch_c2c(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
+                      int mode, hipStream_t stream);
+
+// Batched real-to-complex FFT: N = 2*st.n reals per channel -> N/2+1
+// (outmode 0) or N (outmode 1) bins per channel.
+hipError_t launch_r2c(const float *x, float *y, const float *tw, const float *sf, const StageTable &st,
+                      uint32_t numChannels, uint32_t outmode, hipStream_t stream);
+
+// Batched complex-to-real inverse FFT producing N = 2*st.n reals per channel
+// from N/2+1 (inMode 0) or N (inMode 1) bins per channel. postdiv selects the
+// single-channel scaling (division at the store).
+hipError_t launch_c2r(const float *x, float *y, const float *tw, const float *sf, const StageTable &st,
+                      uint32_t numChannels, uint32_t inMode, int postdiv, hipStream_t stream);
+
+// Batched int16 fixed-point FFT (st.n a power of two). shift holds one entry
+// per radix-4 stage.
+hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uint32_t *shift,
+                      const StageTable &st, uint32_t numChannels, hipStream_t stream);
+
+// Largest complex length the single-pass LDS kernels accept.
+uint32_t max_lds_points();
+
+}  // namespace fftlib

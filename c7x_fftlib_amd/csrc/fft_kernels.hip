@@ -1,0 +1,559 @@
+// gfx950 kernels of the FFTLIB fft_c7x hot path: batched mixed-radix fp32
+// complex FFT (forward / inverse), fused real-to-complex and complex-to-real
+// transforms, and the bit-exact int16 fixed-point FFT.
+//
+// Design (single pass, LDS resident): one workgroup owns F whole transforms.
+// It reads them from HBM once with coalesced loads, stages the caller's
+// twiddle table into LDS, runs every decimation-in-frequency stage in LDS
+// (one butterfly per work item, a barrier between stages) and writes the
+// result once, reading LDS in the digit-reversed order so that HBM writes stay
+// coalesced and in natural order. HBM traffic is therefore one read of the
+// input and one write of the output per transform -- the algorithmic minimum --
+// instead of the reference's one streaming pass per radix stage.
+//
+// The stage sequence, twiddle-table layout, radix-4 bit-swapped stores and
+// output permutation follow the reference natural-C kernels
+// (src/fft_c7x/FFTLIB_fft1dBatched_i32fc_c32fc_o32fc/..._cn.c:240-491); the
+// int16 arithmetic follows src/fft_c7x/FFTLIB_fft1dBatched_i16sc_c16sc_o16sc/
+// ..._cn.c:11-338 bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "fft_kernels.h"
+
+namespace fftlib {
+namespace {
+
+constexpr int      kThreads         = 256;
+constexpr uint32_t kTargetLdsPoints = 4096;  // complex data points resident per workgroup
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 sub2(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 w)
+{
+   return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+// -i * z
+__device__ __forceinline__ float2 mul_mi(float2 z) { return make_float2(z.y, -z.x); }
+
+// k / r and k % r for the digit radices 2, 3, 5 (constant divisors).
+__device__ __forceinline__ uint32_t div_digit(uint32_t k, uint32_t r)
+{
+   switch (r) {
+   case 2: return k >> 1;
+   case 3: return k / 3u;
+   case 5: return k / 5u;
+   default: return k / r;
+   }
+}
+
+// Position, in the in-place DIF result, of frequency k.
+__device__ __forceinline__ uint32_t dif_position(uint32_t k, const StageTable &st)
+{
+   if (st.pow2) {
+      return st.log2n ? (__brev(k) >> (32u - st.log2n)) : 0u;
+   }
+   uint32_t pos = 0;
+   for (uint32_t i = 0; i < st.nDigits; ++i) {
+      const uint32_t r  = st.digit[i];
+      const uint32_t qd = div_digit(k, r);
+      pos               = pos * r + (k - qd * r);
+      k                 = qd;
+   }
+   return pos;
+}
+
+// cos/sin of 2*pi/5, 4*pi/5 and 2*pi/3.
+constexpr float kC51 = 0.30901699437494745f;
+constexpr float kS51 = 0.95105651629515353f;
+constexpr float kC52 = -0.80901699437494734f;
+constexpr float kS52 = 0.58778525229247325f;
+constexpr float kS3  = 0.86602540378443871f;
+
+// ---------------------------------------------------------------------------
+// fp32 DIF stages in LDS
+// ---------------------------------------------------------------------------
+// Radix-5: X_m = sum_j x_j w5^{mj}, w5 = exp(-2 pi i / 5), then output m is
+// multiplied by its stage twiddle. Written with the symmetric sums
+// a1 = x1+x4, b1 = x1-x4, a2 = x2+x3, b2 = x2-x3.
+__device__ __forceinline__ void bfly5(float2 *b0, uint32_t q, const float2 *w, uint32_t p)
+{
+   const float2 x0 = b0[0], x1 = b0[q], x2 = b0[2 * q], x3 = b0[3 * q], x4 = b0[4 * q];
+   const float2 a1 = add2(x1, x4), b1 = sub2(x1, x4), a2 = add2(x2, x3), b2 = sub2(x2, x3);
+   const float2 c1 = make_float2(x0.x + kC51 * a1.x + kC52 * a2.x, x0.y + kC51 * a1.y + kC52 * a2.y);
+   const float2 c2 = make_float2(x0.x + kC52 * a1.x + kC51 * a2.x, x0.y + kC52 * a1.y + kC51 * a2.y);
+   const float2 s1 = make_float2(kS51 * b1.x + kS52 * b2.x, kS51 * b1.y + kS52 * b2.y);
+   const float2 s2 = make_float2(kS52 * b1.x - kS51 * b2.x, kS52 * b1.y - kS51 * b2.y);
+   b0[0]     = add2(add2(x0, a1), a2);
+   b0[q]     = cmul(add2(c1, mul_mi(s1)), w[p]);
+   b0[2 * q] = cmul(add2(c2, mul_mi(s2)), w[q + p]);
+   b0[3 * q] = cmul(sub2(c2, mul_mi(s2)), w[2 * q + p]);
+   b0[4 * q] = cmul(sub2(c1, mul_mi(s1)), w[3 * q + p]);
+}
+
+// Radix-3: X1,2 = x0 - a/2 -/+ i s3 b with a = x1+x2, b = x1-x2.
+__device__ __forceinline__ void bfly3(float2 *b0, uint32_t q, const float2 *w, uint32_t p)
+{
+   const float2 x0 = b0[0], x1 = b0[q], x2 = b0[2 * q];
+   const float2 a = add2(x1, x2), b = sub2(x1, x2);
+   const float2 c = make_float2(x0.x - 0.5f * a.x, x0.y - 0.5f * a.y);
+   const float2 s = mul_mi(make_float2(kS3 * b.x, kS3 * b.y));
+   b0[0]     = add2(x0, a);
+   b0[q]     = cmul(add2(c, s), w[p]);
+   b0[2 * q] = cmul(sub2(c, s), w[q + p]);
+}
+
+// Radix-4 with output twiddles; outputs 1 and 2 are stored swapped
+// (reference ..._cn.c:368-381).
+__device__ __forceinline__ void bfly4(float2 *b0, uint32_t q, const float2 *w, uint32_t p)
+{
+   const float2 x0 = b0[0], x1 = b0[q], x2 = b0[2 * q], x3 = b0[3 * q];
+   const float2 s02 = add2(x0, x2), d02 = sub2(x0, x2), s13 = add2(x1, x3), d13 = sub2(x1, x3);
+   b0[0]     = add2(s02, s13);
+   b0[q]     = cmul(sub2(s02, s13), w[q + p]);            // X2 -> slot 1
+   b0[2 * q] = cmul(add2(d02, mul_mi(d13)), w[p]);        // X1 -> slot 2
+   b0[3 * q] = cmul(sub2(d02, mul_mi(d13)), w[2 * q + p]); // X3
+}
+
+__device__ void dif_stages_f32(float2 *data, uint32_t nf, const float2 *twl, const StageTable &st)
+{
+   const uint32_t N = st.n;
+   uint32_t       M = N;
+   for (uint32_t s = 0; s < st.nStages; ++s) {
+      const uint32_t r     = st.radix[s];
+      const uint32_t q     = M / r;
+      const uint32_t nb    = N / r;
+      const uint32_t total = nb * nf;
+      const float2  *w     = twl + st.twOff[s];
+      for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+         const uint32_t f = idx / nb;
+         const uint32_t k = idx - f * nb;
+         float2        *c = data + f * N;
+         if (r == 2) {
+            const float2 a = c[2 * k], b = c[2 * k + 1];
+            c[2 * k]       = add2(a, b);
+            c[2 * k + 1]   = sub2(a, b);
+         }
+         else {
+            const uint32_t dft = k / q;
+            const uint32_t p   = k - dft * q;
+            float2        *b0  = c + dft * M + p;
+            if (r == 4)
+               bfly4(b0, q, w, p);
+            else if (r == 5)
+               bfly5(b0, q, w, p);
+            else
+               bfly3(b0, q, w, p);
+         }
+      }
+      __syncthreads();
+      M = q;
+   }
+}
+
+__device__ __forceinline__ void load_twiddles(float2 *twl, const float2 *tw, uint32_t count)
+{
+   for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) twl[i] = tw[i];
+}
+
+// ---------------------------------------------------------------------------
+// fp32 kernels
+// ---------------------------------------------------------------------------
+template <int MODE, bool TWLDS>
+__global__ void __launch_bounds__(kThreads) c2c_lds_kernel(const float2 *__restrict__ x, float2 *__restrict__ y,
+                                                           const float2 *__restrict__ tw, StageTable st,
+                                                           uint32_t numChannels, uint32_t F)
+{
+   extern __shared__ float2 smem[];
+   const uint32_t N     = st.n;
+   const uint32_t first = blockIdx.x * F;
+   const uint32_t nf    = min(F, numChannels - first);
+   float2        *data  = smem;
+   const float2  *twl   = TWLDS ? smem + F * N : tw;
+   const size_t   base  = (size_t) first * N;
+   const float    scale = 1.0f / (float) N;
+
+   if (TWLDS) load_twiddles(smem + F * N, tw, st.twCount);
+   for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) {
+      float2 v = x[base + i];
+      if (MODE != C2C_FORWARD) v.y = -v.y;
+      if (MODE == C2C_INVERSE_PRESCALE) {
+         v.x = scale * v.x;
+         v.y = scale * v.y;
+      }
+      data[i] = v;
+   }
+   __syncthreads();
+   dif_stages_f32(data, nf, twl, st);
+   for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) {
+      const uint32_t f = i / N;
+      const uint32_t k = i - f * N;
+      float2         v = data[f * N + dif_position(k, st)];
+      if (MODE == C2C_INVERSE_PRESCALE) v.y = -v.y;
+      if (MODE == C2C_INVERSE_POSTDIV) {
+         v.x = v.x / (float) N;
+         v.y = -(v.y / (float) N);
+      }
+      y[base + i] = v;
+   }
+}
+
+// Real-to-complex: N = 2*Nc reals per channel viewed as Nc complex points,
+// Nc-point FFT in LDS, then the split with factors A[i] = sf[2i..],
+// B[i] = sf[2N-2i-2..] (reference src/fft_c7x/FFTLIB_fft1dBatched_i32f_c32fc_o32fc/
+// ..._cn.c:40-121).
+template <bool TWLDS>
+__global__ void __launch_bounds__(kThreads) r2c_lds_kernel(const float2 *__restrict__ x, float2 *__restrict__ y,
+                                                           const float2 *__restrict__ tw,
+                                                           const float *__restrict__ sf, StageTable st,
+                                                           uint32_t numChannels, uint32_t F, uint32_t outmode)
+{
+   extern __shared__ float2 smem[];
+   const uint32_t Nc    = st.n;
+   const uint32_t N     = 2u * Nc;
+   const uint32_t nOut  = outmode ? N : Nc + 1u;
+   const uint32_t first = blockIdx.x * F;
+   const uint32_t nf    = min(F, numChannels - first);
+   float2        *data  = smem;
+   const float2  *twl   = TWLDS ? smem + F * Nc : tw;
+
+   if (TWLDS) load_twiddles(smem + F * Nc, tw, st.twCount);
+   for (uint32_t i = threadIdx.x; i < nf * Nc; i += blockDim.x) data[i] = x[(size_t) first * Nc + i];
+   __syncthreads();
+   dif_stages_f32(data, nf, twl, st);
+   for (uint32_t idx = threadIdx.x; idx < nf * Nc; idx += blockDim.x) {
+      const uint32_t f   = idx / Nc;
+      const uint32_t i   = idx - f * Nc;
+      const float2  *c   = data + f * Nc;
+      float2        *out = y + (size_t) (first + f) * nOut;
+      const float2   zi  = c[dif_position(i, st)];
+      const float    ar = sf[2u * i], ai = sf[2u * i + 1u];
+      const float    br = sf[2u * N - 2u * i - 2u], bi = sf[2u * N - 2u * i - 1u];
+      float2         t1;
+      if (i == 0u) {
+         t1 = make_float2(zi.x * br + zi.y * bi, zi.x * br - zi.y * bi);
+      }
+      else {
+         const float2 zm = c[dif_position(Nc - i, st)];
+         t1              = make_float2(zm.x * br + zm.y * bi, zm.x * bi - zm.y * br);
+      }
+      const float2 v = make_float2(zi.x * ar - zi.y * ai + t1.x, zi.x * ai + zi.y * ar + t1.y);
+      out[i]         = v;
+      if (outmode && i > 0u) out[N - i] = make_float2(v.x, -v.y);
+      if (i == 0u) out[Nc] = make_float2(zi.x - zi.y, 0.0f);
+   }
+}
+
+// Complex-to-real: Z[i] = X[i] A'[i] + V[i] B'[i] with V from the Hermitian
+// mirror (inMode 0) or the upper half (inMode 1), then the Nc-point inverse
+// (reference src/fft_c7x/FFTLIB_ifft1dBatched_i32fc_c32fc_o32f/..._cn.c:35-87).
+template <bool POSTDIV, bool TWLDS>
+__global__ void __launch_bounds__(kThreads) c2r_lds_kernel(const float2 *__restrict__ x, float2 *__restrict__ y,
+                                                           const float2 *__restrict__ tw,
+                                                           const float *__restrict__ sf, StageTable st,
+                                                           uint32_t numChannels, uint32_t F, uint32_t inMode)
+{
+   extern __shared__ float2 smem[];
+   const uint32_t Nc       = st.n;
+   const uint32_t N        = 2u * Nc;
+   const uint32_t inStride = inMode ? N : Nc + 1u;
+   const uint32_t first    = blockIdx.x * F;
+   const uint32_t nf       = min(F, numChannels - first);
+   float2        *data     = smem;
+   const float2  *twl      = TWLDS ? smem + F * Nc : tw;
+   const float    scale    = 1.0f / (float) Nc;
+
+   if (TWLDS) load_twiddles(smem + F * Nc, tw, st.twCount);
+   for (uint32_t idx = threadIdx.x; idx < nf * Nc; idx += blockDim.x) {
+      const uint32_t f  = idx / Nc;
+      const uint32_t i  = idx - f * Nc;
+      const float2  *in = x + (size_t) (first + f) * inStride;
+      const float2   v0 = in[i];
+      float2         v1;
+      if (inMode) {
+         v1 = in[Nc + i];
+      }
+      else {
+         v1   = in[Nc - i];
+         v1.y = -v1.y;
+      }
+      const float ar = sf[2u * i], ai = sf[2u * i + 1u];
+      const float br = sf[N + 2u * i], bi = sf[N + 2u * i + 1u];
+      float2      z  = make_float2(v0.x * ar - v0.y * ai + (v1.x * br - v1.y * bi),
+                                   v0.x * ai + v0.y * ar + (v1.x * bi + v1.y * br));
+      z.y            = -z.y;
+      if (!POSTDIV) {
+         z.x = scale * z.x;
+         z.y = scale * z.y;
+      }
+      data[idx] = z;
+   }
+   __syncthreads();
+   dif_stages_f32(data, nf, twl, st);
+   for (uint32_t idx = threadIdx.x; idx < nf * Nc; idx += blockDim.x) {
+      const uint32_t f = idx / Nc;
+      const uint32_t n = idx - f * Nc;
+      float2         v = data[f * Nc + dif_position(n, st)];
+      if (POSTDIV) {
+         v.x = v.x / (float) Nc;
+         v.y = -(v.y / (float) Nc);
+      }
+      else {
+         v.y = -v.y;
+      }
+      y[(size_t) first * Nc + idx] = v;
+   }
+}
+
+// ---------------------------------------------------------------------------
+// int16 fixed point
+// ---------------------------------------------------------------------------
+// Values are kept as sign-extended int in registers and packed (re in the low
+// half-word, im in the high half-word -- the memory order of interleaved
+// int16 pairs) in LDS.
+__device__ __forceinline__ int lo16(uint32_t v) { return (int) (short) (v & 0xFFFFu); }
+__device__ __forceinline__ int hi16(uint32_t v) { return (int) (short) (v >> 16); }
+__device__ __forceinline__ uint32_t pack16(int re, int im)
+{
+   return ((uint32_t) re & 0xFFFFu) | (((uint32_t) im & 0xFFFFu) << 16);
+}
+// int -> int16 modulo 2^16 (the reference's int16_t assignment under gcc)
+__device__ __forceinline__ int wrap16(int v) { return (int) (short) v; }
+// real part of the saturating rotation by +j (VCROT90H): -im, -(-32768) -> 32767
+__device__ __forceinline__ int rot90_re(int im) { return im == -32768 ? 32767 : -im; }
+// Q15 product rounding: floor((acc + 2^14) / 2^15) clamped to int16. acc is a
+// sum of two int16*int16 products, |acc| <= 2^31 - 2^15, exact in int32.
+__device__ __forceinline__ int q15_round_sat(int acc)
+{
+   const int r = (acc + 16384) >> 15;
+   return min(max(r, -32768), 32767);
+}
+// Rounding right shift: floor(v / 2^s) plus the last bit shifted out; the
+// bit mask is int16(1 << (s-1)) with shift counts taken modulo 32, so s == 0
+// leaves v unchanged (reference ..._cn.c:36-45 as executed on x86).
+__device__ __forceinline__ int shift_round(int v, uint32_t s)
+{
+   const int mask = (int) (short) (unsigned short) ((1u << ((s - 1u) & 31u)) & 0xFFFFu);
+   const int sh   = v >> (s & 31u);
+   return wrap16((v & mask) != 0 ? sh + 1 : sh);
+}
+
+__global__ void __launch_bounds__(kThreads) i16_lds_kernel(const uint32_t *__restrict__ x, uint32_t *__restrict__ y,
+                                                           const uint32_t *__restrict__ tw,
+                                                           const uint32_t *__restrict__ shift, StageTable st,
+                                                           uint32_t numChannels, uint32_t F)
+{
+   extern __shared__ uint32_t smem16[];
+   const uint32_t N     = st.n;
+   const uint32_t first = blockIdx.x * F;
+   const uint32_t nf    = min(F, numChannels - first);
+   uint32_t      *data  = smem16;
+   uint32_t      *twl   = smem16 + F * N;
+   const size_t   base  = (size_t) first * N;
+
+   for (uint32_t i = threadIdx.x; i < st.twCount; i += blockDim.x) twl[i] = tw[i];
+   for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) data[i] = x[base + i];
+   __syncthreads();
+
+   uint32_t M = N, stage = 0, sh = 0, twOff = 0;
+   while (M > 2u) {
+      sh                  = shift[stage];
+      const uint32_t q    = M >> 2;
+      const uint32_t lq   = 31u - __clz(q);
+      const uint32_t nb   = N >> 2;
+      const uint32_t lnb  = 31u - __clz(nb);
+      const uint32_t *w   = twl + twOff;
+      for (uint32_t idx = threadIdx.x; idx < nb * nf; idx += blockDim.x) {
+         const uint32_t f   = idx >> lnb;
+         const uint32_t k   = idx & (nb - 1u);
+         const uint32_t dft = k >> lq;
+         const uint32_t p   = k & (q - 1u);
+         uint32_t      *b0  = data + f * N + dft * M + p;
+         const uint32_t v0 = b0[0], v1 = b0[q], v2 = b0[2 * q], v3 = b0[3 * q];
+         const int s1r = wrap16(lo16(v0) + lo16(v2)), s1i = wrap16(hi16(v0) + hi16(v2));
+         const int s2r = wrap16(lo16(v1) + lo16(v3)), s2i = wrap16(hi16(v1) + hi16(v3));
+         const int d1r = wrap16(lo16(v0) - lo16(v2)), d1i = wrap16(hi16(v0) - hi16(v2));
+         const int d2r = wrap16(lo16(v1) - lo16(v3)), d2i = wrap16(hi16(v1) - hi16(v3));
+         int       o0r = wrap16(s1r + s2r), o0i = wrap16(s1i + s2i);
+         int       o1r = wrap16(d1r - rot90_re(d2i)), o1i = wrap16(d1i - d2r);
+         int       o2r = wrap16(s1r - s2r), o2i = wrap16(s1i - s2i);
+         int       o3r = wrap16(d1r + rot90_re(d2i)), o3i = wrap16(d1i + d2r);
+         const bool noTw = (M == 4u) || (M == 8u && p == 0u);
+         if (!noTw) {
+            const uint32_t w1 = w[p], w2 = w[q + p], w3 = w[2 * q + p];
+            int            t;
+            t   = q15_round_sat(o1r * lo16(w1) - o1i * hi16(w1));
+            o1i = q15_round_sat(o1r * hi16(w1) + o1i * lo16(w1));
+            o1r = t;
+            t   = q15_round_sat(o2r * lo16(w2) - o2i * hi16(w2));
+            o2i = q15_round_sat(o2r * hi16(w2) + o2i * lo16(w2));
+            o2r = t;
+            t   = q15_round_sat(o3r * lo16(w3) - o3i * hi16(w3));
+            o3i = q15_round_sat(o3r * hi16(w3) + o3i * lo16(w3));
+            o3r = t;
+         }
+         if (M != 8u) {
+            o0r = shift_round(o0r, sh);
+            o0i = shift_round(o0i, sh);
+            o1r = shift_round(o1r, sh);
+            o1i = shift_round(o1i, sh);
+            o2r = shift_round(o2r, sh);
+            o2i = shift_round(o2i, sh);
+            o3r = shift_round(o3r, sh);
+            o3i = shift_round(o3i, sh);
+         }
+         b0[0]     = pack16(o0r, o0i);
+         b0[q]     = pack16(o2r, o2i);
+         b0[2 * q] = pack16(o1r, o1i);
+         b0[3 * q] = pack16(o3r, o3i);
+      }
+      __syncthreads();
+      twOff += 3u * q;
+      M = q;
+      ++stage;
+   }
+   if (M == 2u) {
+      const uint32_t nb = N >> 1;
+      for (uint32_t idx = threadIdx.x; idx < nb * nf; idx += blockDim.x) {
+         const uint32_t f  = idx / nb;
+         const uint32_t k  = idx - f * nb;
+         uint32_t      *a  = data + f * N + 2u * k;
+         const uint32_t u0 = a[0], u1 = a[1];
+         a[0] = pack16(shift_round(wrap16(lo16(u0) + lo16(u1)), sh), shift_round(wrap16(hi16(u0) + hi16(u1)), sh));
+         a[1] = pack16(shift_round(wrap16(lo16(u0) - lo16(u1)), sh), shift_round(wrap16(hi16(u0) - hi16(u1)), sh));
+      }
+      __syncthreads();
+   }
+   for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) {
+      const uint32_t f = i / N;
+      const uint32_t k = i - f * N;
+      y[base + i]      = data[f * N + dif_position(k, st)];
+   }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+uint32_t g_maxLdsBytes = 0;
+
+uint32_t max_lds_bytes()
+{
+   if (g_maxLdsBytes == 0) {
+      int dev = 0, v = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+         g_maxLdsBytes = (uint32_t) v;
+      else
+         g_maxLdsBytes = 65536u;
+   }
+   return g_maxLdsBytes;
+}
+
+// Launch geometry: transforms per workgroup (F) and LDS bytes. The twiddle
+// table is staged in LDS when it fits next to the data, otherwise the stages
+// read it from global memory (L1/L2 resident).
+struct Geometry {
+   uint32_t F;
+   size_t   ldsBytes;
+   bool     twInLds;
+};
+
+bool pick_geometry(uint32_t n, uint32_t elemBytes, uint32_t twCount, uint32_t numChannels, Geometry *g)
+{
+   const size_t limit = max_lds_bytes();
+   uint32_t     F     = n >= kTargetLdsPoints ? 1u : kTargetLdsPoints / n;
+   if (F > numChannels) F = numChannels;
+   if (F == 0) F = 1;
+   for (int withTw = 1; withTw >= 0; --withTw) {
+      uint32_t f = F;
+      while (f > 0 && ((size_t) f * n + (withTw ? twCount : 0u)) * elemBytes > limit) f >>= 1;
+      if (f > 0) {
+         g->F        = f;
+         g->twInLds  = withTw != 0;
+         g->ldsBytes = ((size_t) f * n + (withTw ? twCount : 0u)) * elemBytes;
+         return true;
+      }
+   }
+   return false;
+}
+
+template <typename K>
+hipError_t prepare_lds(K kernel, size_t ldsBytes)
+{
+   if (ldsBytes > 65536) {
+      return hipFuncSetAttribute((const void *) kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) ldsBytes);
+   }
+   return hipSuccess;
+}
+
+template <typename K, typename... Args>
+hipError_t launch(K kernel, const Geometry &g, uint32_t numChannels, hipStream_t stream, Args... args)
+{
+   hipError_t err = prepare_lds(kernel, g.ldsBytes);
+   if (err != hipSuccess) return err;
+   hipLaunchKernelGGL(kernel, dim3((numChannels + g.F - 1) / g.F), dim3(kThreads), g.ldsBytes, stream, args...);
+   return hipGetLastError();
+}
+
+}  // namespace
+
+uint32_t max_lds_points() { return max_lds_bytes() / 16u; }
+
+hipError_t launch_c2c(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
+                      int mode, hipStream_t stream)
+{
+   if (numChannels == 0) return hipSuccess;
+   Geometry g;
+   if (!pick_geometry(st.n, 8u, st.twCount, numChannels, &g)) return hipErrorInvalidValue;
+   const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
+   float2       *yo = (float2 *) y;
+   if (mode == C2C_FORWARD)
+      return g.twInLds ? launch(c2c_lds_kernel<C2C_FORWARD, true>, g, numChannels, stream, xi, yo, wi, st, numChannels, g.F)
+                       : launch(c2c_lds_kernel<C2C_FORWARD, false>, g, numChannels, stream, xi, yo, wi, st, numChannels, g.F);
+   if (mode == C2C_INVERSE_PRESCALE)
+      return g.twInLds ? launch(c2c_lds_kernel<C2C_INVERSE_PRESCALE, true>, g, numChannels, stream, xi, yo, wi, st, numChannels, g.F)
+                       : launch(c2c_lds_kernel<C2C_INVERSE_PRESCALE, false>, g, numChannels, stream, xi, yo, wi, st, numChannels, g.F);
+   return g.twInLds ? launch(c2c_lds_kernel<C2C_INVERSE_POSTDIV, true>, g, numChannels, stream, xi, yo, wi, st, numChannels, g.F)
+                    : launch(c2c_lds_kernel<C2C_INVERSE_POSTDIV, false>, g, numChannels, stream, xi, yo, wi, st, numChannels, g.F);
+}
+
+hipError_t launch_r2c(const float *x, float *y, const float *tw, const float *sf, const StageTable &st,
+                      uint32_t numChannels, uint32_t outmode, hipStream_t stream)
+{
+   if (numChannels == 0) return hipSuccess;
+   Geometry g;
+   if (!pick_geometry(st.n, 8u, st.twCount, numChannels, &g)) return hipErrorInvalidValue;
+   const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
+   float2       *yo = (float2 *) y;
+   return g.twInLds ? launch(r2c_lds_kernel<true>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, outmode)
+                    : launch(r2c_lds_kernel<false>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, outmode);
+}
+
+hipError_t launch_c2r(const float *x, float *y, const float *tw, const float *sf, const StageTable &st,
+                      uint32_t numChannels, uint32_t inMode, int postdiv, hipStream_t stream)
+{
+   if (numChannels == 0) return hipSuccess;
+   Geometry g;
+   if (!pick_geometry(st.n, 8u, st.twCount, numChannels, &g)) return hipErrorInvalidValue;
+   const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
+   float2       *yo = (float2 *) y;
+   if (postdiv)
+      return g.twInLds ? launch(c2r_lds_kernel<true, true>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode)
+                       : launch(c2r_lds_kernel<true, false>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode);
+   return g.twInLds ? launch(c2r_lds_kernel<false, true>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode)
+                    : launch(c2r_lds_kernel<false, false>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode);
+}
+
+hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uint32_t *shift,
+                      const StageTable &st, uint32_t numChannels, hipStream_t stream)
+{
+   if (numChannels == 0) return hipSuccess;
+   Geometry g;
+   if (!pick_geometry(st.n, 4u, st.twCount, numChannels, &g) || !g.twInLds) return hipErrorInvalidValue;
+   return launch(i16_lds_kernel, g, numChannels, stream, (const uint32_t *) x, (uint32_t *) y,
+                 (const uint32_t *) tw, shift, st, numChannels, g.F);
+}
+
+}  // namespace fftlib

@@ -1,0 +1,556 @@
+// C-ABI entry points of the ten fft_c7x kernels: _init writes a GPU plan into
+// the caller's pBlock, _kernel runs the plan on the GPU, _checkParams mirrors
+// the reference's optimized-kernel parameter checks (c7504 variant where the
+// reference has one, c71 otherwise; the reference build picks the c7504
+// sources for the C7504/C7524 devices its published numbers come from).
+//
+// Behaviour beyond the reference: _kernel validates the plan and refuses NULL
+// buffers instead of faulting, and pX is never written (the reference uses it
+// as scratch). The fused real-transform kernels do not write pXFFT.
+#include <string.h>
+
+#include "../../include/fftlib.h"
+#include "fft_kernels.h"
+#include "fftlib_runtime.h"
+
+using namespace fftlib;
+
+namespace {
+
+constexpr uint32_t kMaxComplexPoints = 1u << 20;
+
+bool misaligned(const void *p, uintptr_t a) { return ((uintptr_t) p & (a - 1u)) != 0; }
+
+bool smooth235(uint32_t n)
+{
+   // same loop bounds as the reference checks (at most 32 divisions per prime)
+   for (int i = 0; i < 32 && n % 5u == 0u; ++i) n /= 5u;
+   for (int i = 0; i < 32 && n % 3u == 0u; ++i) n /= 3u;
+   for (int i = 0; i < 32 && n % 2u == 0u; ++i) n /= 2u;
+   return n == 1u;
+}
+
+uint32_t radix4_stages(const StageTable &st)
+{
+   uint32_t c = 0;
+   for (uint32_t s = 0; s < st.nStages; ++s) c += st.radix[s] == 4 ? 1u : 0u;
+   return c;
+}
+
+FFTLIB_STATUS write_plan(void *pBlock, KernelId kernel, uint32_t numPoints, uint32_t complexN, uint32_t numChannels,
+                         uint32_t mode, uint32_t numShifts)
+{
+   if (pBlock == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   Plan p;
+   memset(&p, 0, sizeof p);
+   p.magic       = kPlanMagic;
+   p.kernel      = kernel;
+   p.style       = (uint32_t) current_style();
+   p.numPoints   = numPoints;
+   p.numChannels = numChannels;
+   p.mode        = mode;
+   p.numShifts   = numShifts;
+   if (complexN == 0 || complexN > kMaxComplexPoints) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (!build_stage_table(complexN, &p.st)) return FFTLIB_ERR_INVALID_DIMENSION;
+   if ((kernel == K_I16_BATCHED || kernel == K_I16_SINGLE) && !p.st.pow2) return FFTLIB_ERR_INVALID_DIMENSION;
+   memcpy(pBlock, &p, sizeof p);
+   return FFTLIB_SUCCESS;
+}
+
+// Reads and validates the plan an _init call left in pBlock.
+FFTLIB_STATUS read_plan(const void *pBlock, KernelId kernel, uint32_t numPoints, Plan *p)
+{
+   if (pBlock == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   memcpy(p, pBlock, sizeof *p);
+   if (p->magic != kPlanMagic || p->kernel != (uint32_t) kernel) return FFTLIB_ERR_INVALID_VALUE;
+   if (p->numPoints != numPoints) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (p->style == FFTLIB_FUNCTION_NATC) return FFTLIB_ERR_NOT_IMPLEMENTED;
+   return FFTLIB_SUCCESS;
+}
+
+// ---- launchers bound to plans ---------------------------------------------
+hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
+{
+   int mode = C2C_FORWARD;
+   if (p.kernel == K_IC2C_BATCHED) mode = C2C_INVERSE_PRESCALE;
+   if (p.kernel == K_IC2C_SINGLE) mode = C2C_INVERSE_POSTDIV;
+   return launch_c2c((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st, p.numChannels, mode, s);
+}
+
+hipError_t run_r2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
+{
+   return launch_r2c((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0, p.st,
+                     p.numChannels, p.mode, s);
+}
+
+hipError_t run_c2r(const Plan &p, const DeviceBuffers &d, hipStream_t s)
+{
+   return launch_c2r((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0, p.st,
+                     p.numChannels, p.mode, p.kernel == K_C2R_SINGLE ? 1 : 0, s);
+}
+
+hipError_t run_i16(const Plan &p, const DeviceBuffers &d, hipStream_t s)
+{
+   return launch_i16((const int16_t *) d.in, (int16_t *) d.out, (const int16_t *) d.tw, (const uint32_t *) d.aux0,
+                     p.st, p.numChannels, s);
+}
+
+FFTLIB_STATUS exec_c2c(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, FFTLIB_F32 *pY,
+                       uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   Plan          p;
+   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
+   if (st != FFTLIB_SUCCESS) return st;
+   if (pX == nullptr || pW == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (p.style == FFTLIB_FUNCTION_GPU && (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8)))
+      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   p.numChannels     = numChannels;
+   const size_t data = (size_t) numPoints * numChannels * 2u * sizeof(float);
+   CallBuffers  b{{pX, data}, {pW, (size_t) p.st.twCount * 2u * sizeof(float)}, {nullptr, 0}, pY, data};
+   if (data == 0) return FFTLIB_SUCCESS;
+   return run_on_gpu(p, b, run_c2c);
+}
+
+FFTLIB_STATUS exec_i16(KernelId kernel, const int16_t *pX, const int16_t *pW, int16_t *pY, const uint32_t *pShift,
+                       uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   Plan          p;
+   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
+   if (st != FFTLIB_SUCCESS) return st;
+   if (pX == nullptr || pW == nullptr || pY == nullptr || pShift == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (p.style == FFTLIB_FUNCTION_GPU &&
+       (misaligned(pX, 4) || misaligned(pW, 4) || misaligned(pY, 4) || misaligned(pShift, 4)))
+      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   const uint32_t need = radix4_stages(p.st);
+   if (p.numShifts < need) return FFTLIB_ERR_INVALID_DIMENSION;
+   p.numChannels     = numChannels;
+   const size_t data = (size_t) numPoints * numChannels * 2u * sizeof(int16_t);
+   CallBuffers  b{{pX, data},
+                 {pW, (size_t) p.st.twCount * 2u * sizeof(int16_t)},
+                 {pShift, (size_t) need * sizeof(uint32_t)},
+                 pY,
+                 data};
+   if (data == 0) return FFTLIB_SUCCESS;
+   return run_on_gpu(p, b, run_i16);
+}
+
+FFTLIB_STATUS exec_r2c(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, const FFTLIB_F32 *pSf,
+                       FFTLIB_F32 *pY, uint32_t numPoints, uint32_t numChannels, uint32_t outmode, void *pBlock)
+{
+   Plan          p;
+   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
+   if (st != FFTLIB_SUCCESS) return st;
+   if (outmode > 1u) return FFTLIB_ERR_INVALID_OUTMODE;
+   if (pX == nullptr || pW == nullptr || pSf == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (p.style == FFTLIB_FUNCTION_GPU &&
+       (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pSf, 4)))
+      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   p.numChannels       = numChannels;
+   p.mode              = outmode;
+   const uint32_t nOut = outmode ? numPoints : numPoints / 2u + 1u;
+   const size_t   in   = (size_t) numPoints * numChannels * sizeof(float);
+   const size_t   out  = (size_t) nOut * numChannels * 2u * sizeof(float);
+   CallBuffers    b{{pX, in},
+                 {pW, (size_t) p.st.twCount * 2u * sizeof(float)},
+                 {pSf, (size_t) numPoints * 2u * sizeof(float)},
+                 pY,
+                 out};
+   if (in == 0) return FFTLIB_SUCCESS;
+   return run_on_gpu(p, b, run_r2c);
+}
+
+FFTLIB_STATUS exec_c2r(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, const FFTLIB_F32 *pSf,
+                       FFTLIB_F32 *pY, uint32_t numPoints, uint32_t numChannels, uint32_t inMode, void *pBlock)
+{
+   Plan          p;
+   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
+   if (st != FFTLIB_SUCCESS) return st;
+   if (inMode > 1u) return FFTLIB_ERR_INVALID_VALUE;
+   if (pX == nullptr || pW == nullptr || pSf == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (p.style == FFTLIB_FUNCTION_GPU &&
+       (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pSf, 4)))
+      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   p.numChannels      = numChannels;
+   p.mode             = inMode;
+   const uint32_t nIn = inMode ? numPoints : numPoints / 2u + 1u;
+   const size_t   in  = (size_t) nIn * numChannels * 2u * sizeof(float);
+   const size_t   out = (size_t) numPoints * numChannels * sizeof(float);
+   CallBuffers    b{{pX, in},
+                 {pW, (size_t) p.st.twCount * 2u * sizeof(float)},
+                 {pSf, (size_t) numPoints * 2u * sizeof(float)},
+                 pY,
+                 out};
+   if (out == 0) return FFTLIB_SUCCESS;
+   return run_on_gpu(p, b, run_c2r);
+}
+
+// ---- parameter checks (reference _ci checkParams) ---------------------------
+// fp32 C2C batched, forward and inverse: $C2CB/c7504/..._ci.cpp:2806-2852 and
+// $IC2CB/c7504/..._ci.cpp:2898-2946.
+FFTLIB_STATUS check_c2c_batched(const void *pX, const FFTLIB_bufParams1D_t *bx, const void *pW,
+                                const FFTLIB_bufParams1D_t *bw, const void *pY, const FFTLIB_bufParams1D_t *by,
+                                uint32_t numPoints, const void *pBlock)
+{
+   if (pX == nullptr || pW == nullptr || pY == nullptr || pBlock == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (bx->dim_x != by->dim_x) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (bw->dim_x != numPoints * 2u) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (bx->data_type != FFTLIB_FLOAT32 || bw->data_type != FFTLIB_FLOAT32 || by->data_type != FFTLIB_FLOAT32)
+      return FFTLIB_ERR_INVALID_TYPE;
+   if (misaligned(pX, 16)) return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   return smooth235(numPoints) ? FFTLIB_SUCCESS : FFTLIB_ERR_INVALID_DIMENSION;
+}
+
+// Single-channel fp32 checks (C2C1, IC2C1, R2C1, C2R1): X, W and Y descriptors
+// share dim_x; N = dim_x / 2 must be 2/3/5-smooth ($C2C1/c7504/..._ci.cpp:
+// 2279-2323, $IC2C1/c7504/..._ci.cpp:2381-2425, $R2C1/c71/..._ci.cpp:277-322,
+// $C2R1/c71/..._ci.cpp:166-210).
+FFTLIB_STATUS check_single_f32(const void *pX, const FFTLIB_bufParams1D_t *bx, const void *pW,
+                               const FFTLIB_bufParams1D_t *bw, const void *pY, const FFTLIB_bufParams1D_t *by)
+{
+   if (pX == nullptr || pW == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (bx->dim_x != bw->dim_x || bx->dim_x != by->dim_x) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (bx->data_type != FFTLIB_FLOAT32 || bw->data_type != FFTLIB_FLOAT32 || by->data_type != FFTLIB_FLOAT32)
+      return FFTLIB_ERR_INVALID_TYPE;
+   if (misaligned(pX, 16)) return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   return smooth235(bx->dim_x / 2u) ? FFTLIB_SUCCESS : FFTLIB_ERR_INVALID_DIMENSION;
+}
+
+// Batched real transforms: X, W, Y share dim_x; N/2 must be smooth; R2C also
+// validates outmode ($R2CB/c71/..._ci.cpp:442-517, $C2RB/c71/..._ci.cpp:
+// 299-346).
+FFTLIB_STATUS check_real_batched(const void *pX, const FFTLIB_bufParams1D_t *bx, const void *pW,
+                                 const FFTLIB_bufParams1D_t *bw, const void *pY, const FFTLIB_bufParams1D_t *by,
+                                 uint32_t numPoints, int checkOutmode, uint32_t outmode)
+{
+   if (pX == nullptr || pW == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (bx->dim_x != bw->dim_x || bx->dim_x != by->dim_x) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (bx->data_type != FFTLIB_FLOAT32 || bw->data_type != FFTLIB_FLOAT32 || by->data_type != FFTLIB_FLOAT32)
+      return FFTLIB_ERR_INVALID_TYPE;
+   if (misaligned(pX, 16)) return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   if (checkOutmode && outmode != 0u && outmode != 1u) return FFTLIB_ERR_INVALID_OUTMODE;
+   return smooth235(numPoints / 2u) ? FFTLIB_SUCCESS : FFTLIB_ERR_INVALID_DIMENSION;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ===== fp32 C2C batched forward =============================================
+FFTLIB_STATUS FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                         FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                         FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                         uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   return write_plan(pBlock, K_C2C_BATCHED, numPoints, numPoints, numChannels, 0, 0);
+}
+
+FFTLIB_STATUS FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_kernel(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                           FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                           FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                           uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   return exec_c2c(K_C2C_BATCHED, pX, pW, pY, numPoints, numChannels, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                                FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                                FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                                uint32_t numPoints, uint32_t numChannels,
+                                                                void *pBlock)
+{
+   return check_c2c_batched(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY, numPoints, pBlock);
+}
+
+// ===== fp32 C2C batched inverse =============================================
+FFTLIB_STATUS FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                          FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                          FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                          uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   return write_plan(pBlock, K_IC2C_BATCHED, numPoints, numPoints, numChannels, 0, 0);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc_kernel(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                            FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                            FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                            uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   return exec_c2c(K_IC2C_BATCHED, pX, pW, pY, numPoints, numChannels, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                                 FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                                 FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                                 uint32_t numPoints, uint32_t numChannels,
+                                                                 void *pBlock)
+{
+   return check_c2c_batched(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY, numPoints, pBlock);
+}
+
+// ===== fp32 C2C single channel forward / inverse ============================
+FFTLIB_STATUS FFTLIB_fft1d_i32fc_c32fc_o32fc_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX, FFTLIB_F32 *pW,
+                                                  FFTLIB_bufParams1D_t *bufParamsW, FFTLIB_F32 *pY,
+                                                  FFTLIB_bufParams1D_t *bufParamsY, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   const uint32_t n = bufParamsX->dim_x >> 1;
+   return write_plan(pBlock, K_C2C_SINGLE, n, n, 1, 0, 0);
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i32fc_c32fc_o32fc_kernel(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX, FFTLIB_F32 *pW,
+                                                    FFTLIB_bufParams1D_t *bufParamsW, FFTLIB_F32 *pY,
+                                                    FFTLIB_bufParams1D_t *bufParamsY, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   return exec_c2c(K_C2C_SINGLE, pX, pW, pY, bufParamsX->dim_x >> 1, 1, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i32fc_c32fc_o32fc_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                         FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                         FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                         void *pBlock)
+{
+   return check_single_f32(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1d_i32fc_c32fc_o32fc_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX, FFTLIB_F32 *pW,
+                                                   FFTLIB_bufParams1D_t *bufParamsW, FFTLIB_F32 *pY,
+                                                   FFTLIB_bufParams1D_t *bufParamsY, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   const uint32_t n = bufParamsX->dim_x >> 1;
+   return write_plan(pBlock, K_IC2C_SINGLE, n, n, 1, 0, 0);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1d_i32fc_c32fc_o32fc_kernel(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX, FFTLIB_F32 *pW,
+                                                     FFTLIB_bufParams1D_t *bufParamsW, FFTLIB_F32 *pY,
+                                                     FFTLIB_bufParams1D_t *bufParamsY, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   return exec_c2c(K_IC2C_SINGLE, pX, pW, pY, bufParamsX->dim_x >> 1, 1, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1d_i32fc_c32fc_o32fc_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                          FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                          FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                          void *pBlock)
+{
+   return check_single_f32(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY);
+}
+
+// ===== int16 batched / single ===============================================
+FFTLIB_STATUS FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_init(int16_t *pX, FFTLIB_bufParams1D_t *bufParamsX, int16_t *pW,
+                                                         FFTLIB_bufParams1D_t *bufParamsW, int16_t *pY,
+                                                         FFTLIB_bufParams1D_t *bufParamsY, uint32_t *pShift,
+                                                         FFTLIB_bufParams1D_t *bufParamsShift, uint32_t numPoints,
+                                                         uint32_t numChannels, void *pBlock)
+{
+   const uint32_t ns = bufParamsShift ? bufParamsShift->dim_x : 0u;
+   return write_plan(pBlock, K_I16_BATCHED, numPoints, numPoints, numChannels, 0, ns);
+}
+
+FFTLIB_STATUS FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_kernel(int16_t *pX, FFTLIB_bufParams1D_t *bufParamsX, int16_t *pW,
+                                                           FFTLIB_bufParams1D_t *bufParamsW, int16_t *pY,
+                                                           FFTLIB_bufParams1D_t *bufParamsY, uint32_t *pShift,
+                                                           FFTLIB_bufParams1D_t *bufParamsShift, uint32_t numPoints,
+                                                           uint32_t numChannels, void *pBlock)
+{
+   return exec_i16(K_I16_BATCHED, pX, pW, pY, pShift, numPoints, numChannels, pBlock);
+}
+
+// $I16B/c7504/..._ci.cpp:794-840: only the NULL checks are active.
+FFTLIB_STATUS FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_checkParams(int16_t *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                                int16_t *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                                int16_t *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                                uint32_t *pShift, FFTLIB_bufParams1D_t *bufParamsShift,
+                                                                uint32_t numPoints, uint32_t numChannels,
+                                                                void *pBlock)
+{
+   if (pX == nullptr || pW == nullptr || pY == nullptr || pShift == nullptr || pBlock == nullptr)
+      return FFTLIB_ERR_NULL_POINTER;
+   return FFTLIB_SUCCESS;
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i16sc_c16sc_o16sc_init(int16_t *pX, FFTLIB_bufParams1D_t *bufParamsX, int16_t *pW,
+                                                  FFTLIB_bufParams1D_t *bufParamsW, int16_t *pY,
+                                                  FFTLIB_bufParams1D_t *bufParamsY, uint32_t *pShift,
+                                                  FFTLIB_bufParams1D_t *bufParamsShift, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   const uint32_t n  = bufParamsX->dim_x >> 1;
+   const uint32_t ns = bufParamsShift ? bufParamsShift->dim_x : 0u;
+   return write_plan(pBlock, K_I16_SINGLE, n, n, 1, 0, ns);
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i16sc_c16sc_o16sc_kernel(int16_t *pX, FFTLIB_bufParams1D_t *bufParamsX, int16_t *pW,
+                                                    FFTLIB_bufParams1D_t *bufParamsW, int16_t *pY,
+                                                    FFTLIB_bufParams1D_t *bufParamsY, uint32_t *pShift,
+                                                    FFTLIB_bufParams1D_t *bufParamsShift, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   return exec_i16(K_I16_SINGLE, pX, pW, pY, pShift, bufParamsX->dim_x >> 1, 1, pBlock);
+}
+
+// $I16/c7504/..._ci.cpp:659-699
+FFTLIB_STATUS FFTLIB_fft1d_i16sc_c16sc_o16sc_checkParams(int16_t *pX, FFTLIB_bufParams1D_t *bufParamsX, int16_t *pW,
+                                                         FFTLIB_bufParams1D_t *bufParamsW, int16_t *pY,
+                                                         FFTLIB_bufParams1D_t *bufParamsY, uint32_t *pShift,
+                                                         FFTLIB_bufParams1D_t *bufParamsShift, void *pBlock)
+{
+   if (pX == nullptr || pW == nullptr || pY == nullptr || pShift == nullptr || pBlock == nullptr)
+      return FFTLIB_ERR_NULL_POINTER;
+   if (bufParamsX->dim_x != bufParamsW->dim_x || bufParamsX->dim_x != bufParamsY->dim_x)
+      return FFTLIB_ERR_INVALID_DIMENSION;
+   if (bufParamsX->dim_x < 64u * 2u) return FFTLIB_ERR_INVALID_DIMENSION;
+   if (bufParamsX->data_type != FFTLIB_INT16 || bufParamsW->data_type != FFTLIB_INT16 ||
+       bufParamsY->data_type != FFTLIB_INT16)
+      return FFTLIB_ERR_INVALID_TYPE;
+   if (bufParamsShift->data_type != FFTLIB_UINT32) return FFTLIB_ERR_INVALID_TYPE;
+   if (misaligned(pX, 16)) return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+   const uint32_t pow4 = 1u << ((bufParamsShift->dim_x * 2u) & 31u);
+   if (pow4 * 2u != bufParamsX->dim_x && pow4 * 4u != bufParamsX->dim_x) return FFTLIB_ERR_INVALID_DIMENSION;
+   return FFTLIB_SUCCESS;
+}
+
+// ===== fp32 R2C batched / single ============================================
+FFTLIB_STATUS FFTLIB_fft1dBatched_i32f_c32fc_o32fc_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                        FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                        FFTLIB_F32 *pXFFT, FFTLIB_bufParams1D_t *bufParamsXFFT,
+                                                        FFTLIB_F32 *pSf, FFTLIB_bufParams1D_t *bufParamsSf,
+                                                        FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                        uint32_t numPoints, uint32_t numChannels, uint8_t outmode,
+                                                        void *pBlock)
+{
+   if (outmode > 1u) return FFTLIB_ERR_INVALID_OUTMODE;
+   if (numPoints & 1u) return FFTLIB_ERR_INVALID_DIMENSION;
+   return write_plan(pBlock, K_R2C_BATCHED, numPoints, numPoints / 2u, numChannels, outmode, 0);
+}
+
+FFTLIB_STATUS FFTLIB_fft1dBatched_i32f_c32fc_o32fc_kernel(FFTLIB_F32 *FFTLIB_RESTRICT pX,
+                                                          FFTLIB_bufParams1D_t *bufParamsX,
+                                                          FFTLIB_F32 *FFTLIB_RESTRICT pW,
+                                                          FFTLIB_bufParams1D_t *bufParamsW,
+                                                          FFTLIB_F32 *FFTLIB_RESTRICT pXFFT,
+                                                          FFTLIB_bufParams1D_t *bufParamsXFFT,
+                                                          FFTLIB_F32 *FFTLIB_RESTRICT pSf,
+                                                          FFTLIB_bufParams1D_t *bufParamsSf,
+                                                          FFTLIB_F32 *FFTLIB_RESTRICT pY,
+                                                          FFTLIB_bufParams1D_t *bufParamsY, uint32_t numPoints,
+                                                          uint32_t numChannels, uint8_t outmode, void *pBlock)
+{
+   return exec_r2c(K_R2C_BATCHED, pX, pW, pSf, pY, numPoints, numChannels, outmode, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_fft1dBatched_i32f_c32fc_o32fc_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                               FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                               FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                               uint32_t numPoints, uint32_t numChannels,
+                                                               uint8_t outmode, void *pBlock)
+{
+   return check_real_batched(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY, numPoints, 1, outmode);
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i32f_c32fc_o32fc_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX, FFTLIB_F32 *pW,
+                                                 FFTLIB_bufParams1D_t *bufParamsW, FFTLIB_F32 *pXFFT,
+                                                 FFTLIB_bufParams1D_t *bufParamsXFFT, FFTLIB_F32 *pSf,
+                                                 FFTLIB_bufParams1D_t *bufParamsSf, FFTLIB_F32 *pY,
+                                                 FFTLIB_bufParams1D_t *bufParamsY, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   const uint32_t n = bufParamsX->dim_x;
+   if (n & 1u) return FFTLIB_ERR_INVALID_DIMENSION;
+   return write_plan(pBlock, K_R2C_SINGLE, n, n / 2u, 1, 1, 0);
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i32f_c32fc_o32fc_kernel(FFTLIB_F32 *FFTLIB_RESTRICT pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                   FFTLIB_F32 *FFTLIB_RESTRICT pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                   FFTLIB_F32 *FFTLIB_RESTRICT pXFFT,
+                                                   FFTLIB_bufParams1D_t *bufParamsXFFT,
+                                                   FFTLIB_F32 *FFTLIB_RESTRICT pSf, FFTLIB_bufParams1D_t *bufParamsSf,
+                                                   FFTLIB_F32 *FFTLIB_RESTRICT pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                   void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   return exec_r2c(K_R2C_SINGLE, pX, pW, pSf, pY, bufParamsX->dim_x, 1, 1, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_fft1d_i32f_c32fc_o32fc_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                        FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                        FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                        void *pBlock)
+{
+   return check_single_f32(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY);
+}
+
+// ===== fp32 C2R batched / single ============================================
+FFTLIB_STATUS FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                         FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                         FFTLIB_F32 *pXFFT, FFTLIB_bufParams1D_t *bufParamsXFFT,
+                                                         FFTLIB_F32 *pSf, FFTLIB_bufParams1D_t *bufParamsSf,
+                                                         FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                         uint32_t numPoints, uint32_t numChannels, uint8_t inMode,
+                                                         void *pBlock)
+{
+   if (inMode > 1u) return FFTLIB_ERR_INVALID_VALUE;
+   if (numPoints & 1u) return FFTLIB_ERR_INVALID_DIMENSION;
+   return write_plan(pBlock, K_C2R_BATCHED, numPoints, numPoints / 2u, numChannels, inMode, 0);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_kernel(FFTLIB_F32 *FFTLIB_RESTRICT pX,
+                                                           FFTLIB_bufParams1D_t *bufParamsX,
+                                                           FFTLIB_F32 *FFTLIB_RESTRICT pW,
+                                                           FFTLIB_bufParams1D_t *bufParamsW,
+                                                           FFTLIB_F32 *FFTLIB_RESTRICT pXFFT,
+                                                           FFTLIB_bufParams1D_t *bufParamsXFFT,
+                                                           FFTLIB_F32 *FFTLIB_RESTRICT pSf,
+                                                           FFTLIB_bufParams1D_t *bufParamsSf,
+                                                           FFTLIB_F32 *FFTLIB_RESTRICT pY,
+                                                           FFTLIB_bufParams1D_t *bufParamsY, uint32_t numPoints,
+                                                           uint32_t numChannels, uint8_t inMode, void *pBlock)
+{
+   return exec_c2r(K_C2R_BATCHED, pX, pW, pSf, pY, numPoints, numChannels, inMode, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                                FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                                FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                                uint32_t numPoints, uint32_t numChannels,
+                                                                uint8_t inMode, void *pBlock)
+{
+   return check_real_batched(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY, numPoints, 0, inMode);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1d_i32fc_c32fc_o32f_init(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX, FFTLIB_F32 *pW,
+                                                  FFTLIB_bufParams1D_t *bufParamsW, FFTLIB_F32 *pXFFT,
+                                                  FFTLIB_bufParams1D_t *bufParamsXFFT, FFTLIB_F32 *pSf,
+                                                  FFTLIB_bufParams1D_t *bufParamsSf, FFTLIB_F32 *pY,
+                                                  FFTLIB_bufParams1D_t *bufParamsY, void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   const uint32_t n = bufParamsX->dim_x;
+   if (n & 1u) return FFTLIB_ERR_INVALID_DIMENSION;
+   return write_plan(pBlock, K_C2R_SINGLE, n, n / 2u, 1, 1, 0);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1d_i32fc_c32fc_o32f_kernel(FFTLIB_F32 *FFTLIB_RESTRICT pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                    FFTLIB_F32 *FFTLIB_RESTRICT pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                    FFTLIB_F32 *FFTLIB_RESTRICT pXFFT,
+                                                    FFTLIB_bufParams1D_t *bufParamsXFFT,
+                                                    FFTLIB_F32 *FFTLIB_RESTRICT pSf,
+                                                    FFTLIB_bufParams1D_t *bufParamsSf,
+                                                    FFTLIB_F32 *FFTLIB_RESTRICT pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                    void *pBlock)
+{
+   if (bufParamsX == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   return exec_c2r(K_C2R_SINGLE, pX, pW, pSf, pY, bufParamsX->dim_x, 1, 1, pBlock);
+}
+
+FFTLIB_STATUS FFTLIB_ifft1d_i32fc_c32fc_o32f_checkParams(FFTLIB_F32 *pX, FFTLIB_bufParams1D_t *bufParamsX,
+                                                         FFTLIB_F32 *pW, FFTLIB_bufParams1D_t *bufParamsW,
+                                                         FFTLIB_F32 *pY, FFTLIB_bufParams1D_t *bufParamsY,
+                                                         void *pBlock)
+{
+   return check_single_f32(pX, bufParamsX, pW, bufParamsW, pY, bufParamsY);
+}
+
+}  // extern "C"

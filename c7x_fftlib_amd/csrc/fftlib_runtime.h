@@ -1,0 +1,50 @@
+// Process-wide GPU runtime state of the C-ABI layer: selected function style,
+// HIP stream, completion mode, last error text, and the device staging
+// buffers used when callers hand over host memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/FFTLIB_types.h"
+#include "fft_plan.h"
+
+namespace fftlib {
+
+// One buffer of a transform call: caller pointer and byte count. `out`
+// buffers are written back to the caller when staged.
+struct IoBuffer {
+   const void *ptr;
+   size_t      bytes;
+};
+
+struct CallBuffers {
+   IoBuffer in;     // transform input
+   IoBuffer tw;     // twiddle table
+   IoBuffer aux0;   // split factors (real transforms) or shift vector (int16)
+   void    *out;    // transform output
+   size_t   outBytes;
+};
+
+// Device pointers handed to a launcher.
+struct DeviceBuffers {
+   const void *in;
+   const void *tw;
+   const void *aux0;
+   void       *out;
+};
+
+using LaunchFn = hipError_t (*)(const Plan &plan, const DeviceBuffers &d, hipStream_t stream);
+
+// Runs `launch` with the caller's buffers according to the plan's style:
+// FFTLIB_FUNCTION_GPU uses them in place; FFTLIB_FUNCTION_OPTIMIZED stages
+// them through device memory. Returns a FFTLIB status and records HIP errors.
+FFTLIB_STATUS run_on_gpu(const Plan &plan, const CallBuffers &b, LaunchFn launch);
+
+FFTLIB_FUNCTION_STYLE current_style();
+hipStream_t           current_stream();
+bool                  synchronous();
+void                  set_last_error(const char *what, hipError_t err);
+
+}  // namespace fftlib

@@ -1,0 +1,224 @@
+"""Calls into the product library the way the reference test drivers do:
+generate tables with the library's helpers, _init into a PBLOCK_SIZE buffer,
+then _kernel.
+
+Two memory styles are covered:
+  * "gpu": FFTLIB_FUNCTION_GPU with torch device tensors (in-place on the GPU);
+  * "host": FFTLIB_FUNCTION_OPTIMIZED with numpy host arrays (library stages).
+Every function returns (status, numpy result).
+"""
+import ctypes
+
+import numpy as np
+
+import c7x_fftlib_amd as F
+
+
+def lib():
+    return F.load()
+
+
+def _bp(dtype, n):
+    return ctypes.byref(F.BufParams1D(dtype, n))
+
+
+def pblock(name):
+    return ctypes.create_string_buffer(F.PBLOCK_SIZE[name])
+
+
+def tw_c2c(n):
+    w = np.zeros(2 * n, np.float32)
+    lib().FFTLIB_tw_gen_c2c(w.ctypes.data, n)
+    return w
+
+
+def tw_i16(n):
+    w = np.zeros(2 * n, np.int16)
+    lib().FFTLIB_tw_gen_i16sc(w.ctypes.data, n)
+    return w
+
+
+def tw_r2c(n):
+    w = np.zeros(max(n, 2), np.float32)
+    lib().tw_gen_r2c(w.ctypes.data, n)
+    return w[:n]
+
+
+def sf_r2c(n):
+    s = np.zeros(2 * n, np.float32)
+    lib().split_factor_gen_r2c(s.ctypes.data, n)
+    return s
+
+
+def tw_c2r(n):
+    w = np.zeros(max(n, 2), np.float32)
+    lib().ifft_c2r_tw_gen(w.ctypes.data, n)
+    return w[:n]
+
+
+def sf_c2r(n):
+    s = np.zeros(2 * n, np.float32)
+    lib().ifft_c2r_split_factor_gen(s.ctypes.data, n)
+    return s
+
+
+class _Mem:
+    """Moves numpy arrays to the memory of the selected style."""
+
+    def __init__(self, style):
+        self.style = style
+        if style == "gpu":
+            import torch
+            self.torch = torch
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        if self.style == "gpu":
+            t = self.torch.from_numpy(a.copy()).to("cuda:0")
+            return t, t.data_ptr()
+        return a, a.ctypes.data
+
+    def empty(self, count, dtype):
+        if self.style == "gpu":
+            tdt = {np.float32: self.torch.float32, np.int16: self.torch.int16}[dtype]
+            t = self.torch.full((count,), -7, dtype=tdt, device="cuda:0")
+            return t, t.data_ptr()
+        a = np.full(count, -7, dtype)
+        return a, a.ctypes.data
+
+    def get(self, obj):
+        if self.style == "gpu":
+            self.torch.cuda.synchronize()
+            return obj.cpu().numpy()
+        return obj
+
+
+def _set_style(style):
+    st = F.FFTLIB_FUNCTION_GPU if style == "gpu" else F.FFTLIB_FUNCTION_OPTIMIZED
+    assert lib().FFTLIB_setFunctionStyle(st) == F.FFTLIB_SUCCESS
+    lib().FFTLIB_GPU_setSynchronous(1)
+
+
+def c2c(x, n, b, inverse=False, single=False, style="gpu", w=None):
+    """x: complex64 (b, n). Returns (status, complex64 (b, n))."""
+    _set_style(style)
+    m = _Mem(style)
+    xin = np.ascontiguousarray(np.asarray(x, np.complex64)).view(np.float32).reshape(-1)
+    w = tw_c2c(n) if w is None else w
+    xo, xp = m.put(xin)
+    wo, wp = m.put(w)
+    yo, yp = m.empty(xin.size, np.float32)
+    L = lib()
+    if single:
+        name = "FFTLIB_ifft1d_i32fc_c32fc_o32fc" if inverse else "FFTLIB_fft1d_i32fc_c32fc_o32fc"
+        pb = pblock(name)
+        bp = _bp(F.FFTLIB_FLOAT32, 2 * n)
+        s = getattr(L, name + "_init")(xp, bp, wp, bp, yp, bp, pb)
+        if s == 0:
+            s = getattr(L, name + "_kernel")(xp, bp, wp, bp, yp, bp, pb)
+    else:
+        name = "FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc" if inverse else "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc"
+        pb = pblock(name)
+        bx, bw = _bp(F.FFTLIB_FLOAT32, 2 * n * b), _bp(F.FFTLIB_FLOAT32, 2 * n)
+        s = getattr(L, name + "_init")(xp, bx, wp, bw, yp, bx, n, b, pb)
+        if s == 0:
+            s = getattr(L, name + "_kernel")(xp, bx, wp, bw, yp, bx, n, b, pb)
+    y = m.get(yo)
+    x_after = m.get(xo)
+    assert np.array_equal(np.asarray(x_after).view(np.uint32), xin.view(np.uint32)), "input was modified"
+    return s, np.asarray(y, np.float32).view(np.complex64).reshape(b, n)
+
+
+def i16(x, n, b, shifts, single=False, style="gpu", w=None):
+    """x: int16 (b, n, 2). Returns (status, int16 (b, n, 2))."""
+    _set_style(style)
+    m = _Mem(style)
+    xin = np.ascontiguousarray(x, np.int16).reshape(-1)
+    w = tw_i16(n) if w is None else w
+    sh = np.ascontiguousarray(shifts, np.uint32)
+    xo, xp = m.put(xin)
+    wo, wp = m.put(w)
+    if style == "gpu":
+        import torch
+        sht = torch.from_numpy(sh.view(np.int32).copy()).to("cuda:0")
+        shp = sht.data_ptr()
+    else:
+        shp = sh.ctypes.data
+    yo, yp = m.empty(xin.size, np.int16)
+    L = lib()
+    bs = _bp(F.FFTLIB_UINT32, sh.size)
+    if single:
+        name = "FFTLIB_fft1d_i16sc_c16sc_o16sc"
+        pb = pblock(name)
+        bp = _bp(F.FFTLIB_INT16, 2 * n)
+        s = L.FFTLIB_fft1d_i16sc_c16sc_o16sc_init(xp, bp, wp, bp, yp, bp, shp, bs, pb)
+        if s == 0:
+            s = L.FFTLIB_fft1d_i16sc_c16sc_o16sc_kernel(xp, bp, wp, bp, yp, bp, shp, bs, pb)
+    else:
+        name = "FFTLIB_fft1dBatched_i16sc_c16sc_o16sc"
+        pb = pblock(name)
+        bx, bw = _bp(F.FFTLIB_INT16, 2 * n * b), _bp(F.FFTLIB_INT16, 2 * n)
+        s = L.FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_init(xp, bx, wp, bw, yp, bx, shp, bs, n, b, pb)
+        if s == 0:
+            s = L.FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_kernel(xp, bx, wp, bw, yp, bx, shp, bs, n, b, pb)
+    return s, np.asarray(m.get(yo), np.int16).reshape(b, n, 2)
+
+
+def r2c(x, n, b, outmode=0, single=False, style="gpu"):
+    """x: float32 (b, n). Returns (status, complex64 (b, nOut))."""
+    _set_style(style)
+    m = _Mem(style)
+    nout = n if (outmode or single) else n // 2 + 1
+    xo, xp = m.put(np.ascontiguousarray(x, np.float32).reshape(-1))
+    wo, wp = m.put(tw_r2c(n))
+    so, sp = m.put(sf_r2c(n))
+    fo, fp = m.empty(max(n * b, 1), np.float32)
+    yo, yp = m.empty(2 * nout * b, np.float32)
+    L = lib()
+    bp = _bp(F.FFTLIB_FLOAT32, n * b)
+    if single:
+        name = "FFTLIB_fft1d_i32f_c32fc_o32fc"
+        pb = pblock(name)
+        s = L.FFTLIB_fft1d_i32f_c32fc_o32fc_init(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, pb)
+        if s == 0:
+            s = L.FFTLIB_fft1d_i32f_c32fc_o32fc_kernel(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, pb)
+    else:
+        name = "FFTLIB_fft1dBatched_i32f_c32fc_o32fc"
+        pb = pblock(name)
+        s = L.FFTLIB_fft1dBatched_i32f_c32fc_o32fc_init(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, n, b, outmode, pb)
+        if s == 0:
+            s = L.FFTLIB_fft1dBatched_i32f_c32fc_o32fc_kernel(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, n, b, outmode,
+                                                              pb)
+    return s, np.asarray(m.get(yo), np.float32).view(np.complex64).reshape(b, nout)
+
+
+def c2r(x, n, b, inmode=0, single=False, style="gpu"):
+    """x: complex64 (b, n//2+1) (inMode 0) or (b, n). Returns (status, float32 (b, n))."""
+    _set_style(style)
+    m = _Mem(style)
+    xo, xp = m.put(np.ascontiguousarray(np.asarray(x, np.complex64)).view(np.float32).reshape(-1))
+    wo, wp = m.put(tw_c2r(n))
+    so, sp = m.put(sf_c2r(n))
+    fo, fp = m.empty(max(n * b, 1), np.float32)
+    yo, yp = m.empty(n * b, np.float32)
+    L = lib()
+    bp = _bp(F.FFTLIB_FLOAT32, n * b)
+    if single:
+        pb = pblock("FFTLIB_ifft1d_i32fc_c32fc_o32f")
+        s = L.FFTLIB_ifft1d_i32fc_c32fc_o32f_init(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, pb)
+        if s == 0:
+            s = L.FFTLIB_ifft1d_i32fc_c32fc_o32f_kernel(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, pb)
+    else:
+        pb = pblock("FFTLIB_ifft1dBatched_i32fc_c32fc_o32f")
+        s = L.FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_init(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, n, b, inmode, pb)
+        if s == 0:
+            s = L.FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_kernel(xp, bp, wp, bp, fp, bp, sp, bp, yp, bp, n, b, inmode,
+                                                               pb)
+    return s, np.asarray(m.get(yo), np.float32).reshape(b, n)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))

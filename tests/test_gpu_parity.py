@@ -1,0 +1,144 @@
+"""GPU parity: every fft_c7x entry point, run on the MI355X through the C-ABI,
+against the oracle (CPU restatement of the reference's natural-C kernels) on
+the same seeded inputs.
+
+Bars: int16 bit-exact; fp32 relative L2 <= 1e-6 against the oracle and, in
+addition, the reference drivers' own absolute tolerances (2^-10 for the
+single-channel C2C, 2^-5 for batched C2C/IFFT with +-256 inputs, 2^-3 for
+R2C, 1e-3 for C2R).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import fftlib_calls as G
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-6
+
+
+def rng(seed):
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def cplx(r, b, n, scale=1.0):
+    return (r.uniform(-scale, scale, (b, n)) + 1j * r.uniform(-scale, scale, (b, n))).astype(np.complex64)
+
+
+C2C_SIZES = [1, 2, 3, 4, 5, 8, 15, 16, 30, 60, 64, 100, 120, 128, 243, 256, 960, 1000, 1024, 1536, 3125, 4096, 6250,
+             8192]
+
+
+@pytest.mark.parametrize("style", ["gpu", "host"])
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("n", C2C_SIZES)
+def test_c2c_batched(n, inverse, style):
+    b = 3 if n < 4096 else 2
+    x = cplx(rng(n * 7 + inverse), b, n, 256.0)
+    s, y = G.c2c(x, n, b, inverse=inverse, style=style)
+    assert s == 0, G.F.last_error()
+    ref = oracle.c2c(x, n, b, inverse=inverse)
+    assert G.rel_l2(y, ref) <= REL_TOL
+    tol = 2.0 ** -5 if not inverse else 2.0 ** -5
+    assert np.max(np.abs(y - ref)) <= max(tol, 1e-6 * np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("n", [16, 64, 128, 256, 512, 1024, 2048, 4096, 30, 60, 120, 240, 480, 960, 1920, 3840,
+                               15625])
+def test_c2c_single(n, inverse):
+    x = cplx(rng(n + 11 * inverse), 1, n, 1.0)
+    s, y = G.c2c(x, n, 1, inverse=inverse, single=True)
+    assert s == 0, G.F.last_error()
+    ref = oracle.c2c(x, n, 1, inverse=inverse, single=True)
+    assert G.rel_l2(y, ref) <= REL_TOL
+    assert np.max(np.abs(y - ref)) <= 2.0 ** -10
+
+
+def _i16_input(r, b, n, lo, hi):
+    return r.integers(lo, hi + 1, size=(b, n, 2)).astype(np.int16)
+
+
+I16_CASES = [
+    # (n, b, shifts) -- shapes and shift vectors of the reference test list plus edge shapes
+    (2, 4, [0]), (4, 4, [1]), (8, 3, [1, 1]), (16, 3, [0, 1]), (32, 3, [1, 0, 1]),
+    (64, 32, [0, 1, 1]), (64, 64, [1, 0, 1]), (128, 16, [0, 1, 1]), (128, 256, [1, 0, 0]),
+    (256, 8, [0, 1, 1, 1]), (256, 128, [1, 0, 0, 1]), (512, 4, [0, 1, 1, 1]), (1024, 2, [0, 1, 1, 1, 1]),
+    (2048, 4, [0, 0, 1, 1, 0]), (4096, 2, [0, 1, 1, 0, 0, 1]), (8192, 1, [1, 0, 0, 0, 1, 1]),
+]
+
+
+@pytest.mark.parametrize("style", ["gpu", "host"])
+@pytest.mark.parametrize("n,b,shifts", I16_CASES)
+def test_i16_batched_bit_exact(n, b, shifts, style):
+    r = rng(n * 31 + b)
+    for lo, hi in ((-2048, 2047), (-32768, 32767)):  # config-2 range, then full range (wraps)
+        x = _i16_input(r, b, n, lo, hi)
+        s, y = G.i16(x, n, b, shifts, style=style)
+        assert s == 0, G.F.last_error()
+        ref = oracle.i16(x, n, b, shifts)
+        assert np.array_equal(y, ref), f"range [{lo},{hi}]: {np.count_nonzero(y != ref)} int16 differ"
+
+
+def test_i16_saturation_edges():
+    n, b = 64, 4
+    x = np.full((b, n, 2), -32768, np.int16)
+    x[1, ::2, 1] = 32767
+    x[2] = np.int16(32767)
+    x[3, :, 0] = -32768
+    x[3, :, 1] = 0
+    for shifts in ([0, 0, 0], [1, 1, 1], [2, 0, 3], [16, 17, 31]):
+        s, y = G.i16(x, n, b, shifts)
+        assert s == 0
+        assert np.array_equal(y, oracle.i16(x, n, b, shifts))
+
+
+@pytest.mark.parametrize("n,shifts", [(64, [0, 1, 1]), (256, [1, 0, 1, 1]), (1024, [0, 1, 1, 1, 1]),
+                                      (2048, [1, 0, 1, 0, 1])])
+def test_i16_single_bit_exact(n, shifts):
+    x = _i16_input(rng(n), 1, n, -32768, 32767)
+    s, y = G.i16(x, n, 1, shifts, single=True)
+    assert s == 0, G.F.last_error()
+    assert np.array_equal(y, oracle.i16(x, n, 1, shifts, single=True))
+
+
+R2C_SIZES = [(10, 16), (50, 4), (64, 1), (128, 2), (250, 4), (256, 4), (512, 8), (1024, 4), (1250, 4), (1600, 2),
+             (4096, 3), (6250, 1), (8000, 2), (12800, 1), (16000, 1)]
+
+
+@pytest.mark.parametrize("outmode", [0, 1])
+@pytest.mark.parametrize("n,b", R2C_SIZES)
+def test_r2c_batched(n, b, outmode):
+    x = rng(n + outmode).uniform(-128, 128, (b, n)).astype(np.float32)
+    s, y = G.r2c(x, n, b, outmode=outmode)
+    assert s == 0, G.F.last_error()
+    ref = oracle.r2c(x, n, b, outmode=outmode)
+    assert G.rel_l2(y, ref) <= REL_TOL
+    assert np.max(np.abs(y - ref)) <= 2.0 ** -3
+
+
+@pytest.mark.parametrize("inmode", [0, 1])
+@pytest.mark.parametrize("n,b", R2C_SIZES)
+def test_c2r_batched(n, b, inmode):
+    r = rng(n * 3 + inmode)
+    bins = n // 2 + 1 if inmode == 0 else n
+    x = cplx(r, b, bins, 100.0)
+    s, y = G.c2r(x, n, b, inmode=inmode)
+    assert s == 0, G.F.last_error()
+    ref = oracle.c2r(x, n, b, inmode=inmode)
+    assert G.rel_l2(y, ref) <= REL_TOL
+    assert np.max(np.abs(y - ref)) <= 1e-3
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096, 8192, 10, 50, 250, 1250])
+def test_real_single(n):
+    x = rng(n).uniform(-1, 1, (1, n)).astype(np.float32)
+    s, y = G.r2c(x, n, 1, single=True)
+    assert s == 0, G.F.last_error()
+    ref = oracle.r2c(x, n, 1, single=True)
+    assert G.rel_l2(y, ref) <= REL_TOL
+    s, z = G.c2r(y, n, 1, single=True)
+    assert s == 0, G.F.last_error()
+    refz = oracle.c2r(y, n, 1, single=True)
+    assert G.rel_l2(z, refz) <= REL_TOL
