@@ -1,0 +1,381 @@
+"""Benchmark of the FFTLIB batched FFT hot path on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on):
+  FFTLIB_fft1dBatched_i32fc_c32fc_o32fc, N = 960 = 2^6*3*5, 65536 channels,
+  interleaved complex64 input i.i.d. uniform[-1, 1) from a Philox seed,
+  twiddles from the library's generator, buffers resident in HBM.
+
+A step is one call of the C-ABI _kernel entry point (FFTLIB_FUNCTION_GPU,
+device pointers, asynchronous mode) over the whole batch. W warm-up steps are
+followed by exactly K timed steps bracketed by device synchronisation; HIP
+events recorded on the stream the library launches on give the average launch
+duration. One JSON line is printed:
+  value     = nominal GFLOP/s = 5 N log2(N) B / t
+  roofline  = algorithmic HBM bytes (one read + one write of 8 B per point)
+              / t against the 8 TB/s HBM3E peak, plus measured traffic when a
+              rocprofv3 counter summary for the workload is committed
+  cpu_baseline = the oracle (C restatement of the reference _cn kernel) timed
+              on this host on a bounded sample of the same workload.
+
+Other workloads: --workload i16_256 (int16, N=256, 8192 channels),
+r2c_4096 (R2C then C2R, N=4096, 16384 channels).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Batched C2C FFT GFLOP/s & %HBM peak; split-BF16 MFMA vs FP32 at ≤2e-6 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "copy_503m"])
+    ap.add_argument("--variant", default="fp32", help="fp32 C2C kernel variant (fp32, bf16x3, fp16x2)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target seconds per CPU baseline run")
+    return ap.parse_args()
+
+
+def philox(seed):
+    return np.random.Generator(np.random.Philox(seed))
+
+
+class Workload:
+    """Holds device buffers and a launch closure for one workload."""
+
+    def __init__(self, args, F, lib, torch):
+        self.args, self.F, self.lib, self.torch = args, F, lib, torch
+        getattr(self, "_setup_" + args.workload)()
+
+    # -- batched fp32 C2C, N = 960, B = 65536 --------------------------------
+    def _setup_c2c_960(self):
+        F, lib, torch = self.F, self.lib, self.torch
+        self.n, self.b = 960, 65536
+        n, b = self.n, self.b
+        r = philox(960)
+        host = np.empty((b, n), np.complex64)
+        chunk = 8192
+        for i in range(0, b, chunk):
+            host[i:i + chunk].real = r.uniform(-1, 1, (chunk, n))
+            host[i:i + chunk].imag = r.uniform(-1, 1, (chunk, n))
+        self.host_x = host
+        w = np.zeros(2 * n, np.float32)
+        lib.FFTLIB_tw_gen_c2c(w.ctypes.data, n)
+        self.host_w = w
+        self.x = torch.from_numpy(host.view(np.float32).reshape(-1)).to("cuda:0")
+        self.w = torch.from_numpy(w).to("cuda:0")
+        self.y = torch.empty_like(self.x)
+        self.bp_x = F.BufParams1D(F.FFTLIB_FLOAT32, 2 * n * b)
+        self.bp_w = F.BufParams1D(F.FFTLIB_FLOAT32, 2 * n)
+        self.pblock = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_fft1dBatched_i32fc_c32fc_o32fc"])
+        st = lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_init(self.x.data_ptr(), ctypes.byref(self.bp_x),
+                                                             self.w.data_ptr(), ctypes.byref(self.bp_w),
+                                                             self.y.data_ptr(), ctypes.byref(self.bp_x), n, b,
+                                                             self.pblock)
+        assert st == 0, st
+        self.flops = 5.0 * n * math.log2(n) * b
+        self.bytes = 16.0 * n * b
+        self.dtype = "f32"
+        self.unit = "GFLOP/s"
+        self.config = {"workload": "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc N=960 B=65536 (fp32 C2C)",
+                       "numPoints": n, "numChannels": b, "variant": self.args.variant,
+                       "input": "complex64 uniform[-1,1) Philox seed 960"}
+        self.kernel_name = "c2c"
+        xp, wp, yp = self.x.data_ptr(), self.w.data_ptr(), self.y.data_ptr()
+        bx, bw, pb = ctypes.byref(self.bp_x), ctypes.byref(self.bp_w), self.pblock
+        fn = lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_kernel
+
+        def step():
+            s = fn(xp, bx, wp, bw, yp, bx, n, b, pb)
+            if s != 0:
+                raise RuntimeError(f"kernel status {s}: {F.last_error()}")
+        self.step = step
+
+    def check_c2c_960(self):
+        """Relative L2 of 256 channels against the FP64 DFT and the oracle."""
+        import oracle
+        idx = np.arange(0, self.b, self.b // 256)
+        y = self.y.view(-1, 2 * self.n)[idx].cpu().numpy().view(np.complex64)
+        x = self.host_x[idx]
+        ref64 = np.fft.fft(x.astype(np.complex128), axis=1)
+        err64 = float(np.linalg.norm(y - ref64) / np.linalg.norm(ref64))
+        refo = oracle.c2c(x, self.n, len(idx))
+        erro = float(np.linalg.norm(y - refo) / np.linalg.norm(refo))
+        return {"rel_l2_vs_fp64": err64, "rel_l2_vs_oracle": erro, "checked_channels": int(len(idx))}
+
+    def cpu_c2c_960(self, seconds):
+        import oracle
+        n = self.n
+        w = oracle.tw_c2c(n)
+        probe = 64
+        xs = self.host_x[:probe].view(np.float32).reshape(-1).copy()
+        ys = np.zeros_like(xs)
+        t0 = time.perf_counter()
+        oracle.raw_c2c_batched(xs.copy(), w, ys, n, probe)
+        per = (time.perf_counter() - t0) / probe
+        cnt = int(max(64, min(self.b, seconds / max(per, 1e-9))))
+        src = self.host_x[:cnt].view(np.float32).reshape(-1)
+        times = []
+        for _ in range(5):
+            xin = src.copy()  # the natural-C kernel clobbers its input
+            y = np.zeros_like(xin)
+            t0 = time.perf_counter()
+            oracle.raw_c2c_batched(xin, w, y, n, cnt)
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        flops = 5.0 * n * math.log2(n) * cnt
+        res = {"value": flops / t / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+               "sample": f"oracle FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_cn, N=960, {cnt} of 65536 channels, "
+                         f"median of 5 runs ({t:.3f} s each), input re-copied before each run"}
+        threads = min(os.cpu_count() or 1, 16)
+        if threads > 1:
+            tt = []
+            for _ in range(3):
+                xin = src.copy()
+                y = np.zeros_like(xin)
+                t0 = time.perf_counter()
+                oracle.raw_threads_c2c(xin, w, y, n, cnt, False, threads)
+                tt.append(time.perf_counter() - t0)
+            res["all_cores"] = {"value": flops / float(np.median(tt)) / 1e9, "cores": threads,
+                                "sample": "same sample, channels split over POSIX threads"}
+        return res
+
+    # -- plain device copy of the C2C workload's input size (counter calibration)
+    def _setup_copy_503m(self):
+        torch = self.torch
+        self.n, self.b = 960, 65536
+        count = 2 * self.n * self.b
+        self.x = torch.ones(count, dtype=torch.float32, device="cuda:0")
+        self.y = torch.empty_like(self.x)
+        self.flops = 0.0
+        self.bytes = 2.0 * 4 * count
+        self.dtype = "f32"
+        self.unit = "GB/s"
+        self.config = {"workload": "torch copy_ of 503,316,480 bytes (HBM counter calibration)"}
+        self.kernel_name = "copy"
+
+        def step():
+            self.y.copy_(self.x)
+        self.step = step
+
+    def check_copy_503m(self):
+        return {"copied_equal": bool(torch_equal(self.x, self.y))}
+
+    def cpu_copy_503m(self, seconds):
+        return None
+
+    # -- int16, N = 256, B = 8192 ---------------------------------------------
+    def _setup_i16_256(self):
+        F, lib, torch = self.F, self.lib, self.torch
+        self.n, self.b = 256, 8192
+        n, b = self.n, self.b
+        r = philox(256)
+        host = r.integers(-2048, 2048, size=(b, n, 2)).astype(np.int16)
+        self.host_x = host
+        w = np.zeros(2 * n, np.int16)
+        lib.FFTLIB_tw_gen_i16sc(w.ctypes.data, n)
+        self.shifts = np.array([0, 1, 1, 1], np.uint32)
+        self.x = torch.from_numpy(host.reshape(-1)).to("cuda:0")
+        self.w = torch.from_numpy(w).to("cuda:0")
+        self.sh = torch.from_numpy(self.shifts.view(np.int32)).to("cuda:0")
+        self.y = torch.empty_like(self.x)
+        self.bp_x = F.BufParams1D(F.FFTLIB_INT16, 2 * n * b)
+        self.bp_w = F.BufParams1D(F.FFTLIB_INT16, 2 * n)
+        self.bp_s = F.BufParams1D(F.FFTLIB_UINT32, 4)
+        self.pblock = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_fft1dBatched_i16sc_c16sc_o16sc"])
+        args = (self.x.data_ptr(), ctypes.byref(self.bp_x), self.w.data_ptr(), ctypes.byref(self.bp_w),
+                self.y.data_ptr(), ctypes.byref(self.bp_x), self.sh.data_ptr(), ctypes.byref(self.bp_s), n, b,
+                self.pblock)
+        assert lib.FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_init(*args) == 0
+        self.flops = 5.0 * n * math.log2(n) * b
+        self.bytes = 8.0 * n * b
+        self.dtype = "int16"
+        self.unit = "GOP/s"
+        self.config = {"workload": "FFTLIB_fft1dBatched_i16sc_c16sc_o16sc N=256 B=8192 shifts [0,1,1,1]",
+                       "numPoints": n, "numChannels": b}
+        self.kernel_name = "i16"
+        fn = lib.FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_kernel
+
+        def step():
+            s = fn(*args)
+            if s != 0:
+                raise RuntimeError(f"kernel status {s}: {F.last_error()}")
+        self.step = step
+
+    def check_i16_256(self):
+        import oracle
+        y = self.y.cpu().numpy().reshape(self.b, self.n, 2)
+        ref = oracle.i16(self.host_x, self.n, self.b, self.shifts)
+        return {"bit_exact_vs_oracle": bool(np.array_equal(y, ref)), "checked_channels": self.b}
+
+    def cpu_i16_256(self, seconds):
+        import oracle
+        t = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle.i16(self.host_x, self.n, self.b, self.shifts)
+            t.append(time.perf_counter() - t0)
+        tm = float(np.median(t))
+        return {"value": self.flops / tm / 1e9, "unit": "GOP/s", "cores": 1, "kind": "port",
+                "sample": f"oracle int16 _cn, full workload N=256 B=8192, median of 5 ({tm:.3f} s)"}
+
+    # -- R2C + C2R, N = 4096, B = 16384 ------------------------------------------
+    def _setup_r2c_4096(self):
+        F, lib, torch = self.F, self.lib, self.torch
+        self.n, self.b = 4096, 16384
+        n, b = self.n, self.b
+        r = philox(4096)
+        self.host_x = r.standard_normal((b, n), dtype=np.float32)
+        w = np.zeros(n, np.float32)
+        sf = np.zeros(2 * n, np.float32)
+        lib.tw_gen_r2c(w.ctypes.data, n)
+        lib.split_factor_gen_r2c(sf.ctypes.data, n)
+        wi = np.zeros(n, np.float32)
+        sfi = np.zeros(2 * n, np.float32)
+        lib.ifft_c2r_tw_gen(wi.ctypes.data, n)
+        lib.ifft_c2r_split_factor_gen(sfi.ctypes.data, n)
+        dev = lambda a: torch.from_numpy(a).to("cuda:0")
+        self.x, self.w, self.sf, self.wi, self.sfi = map(dev, (self.host_x.reshape(-1), w, sf, wi, sfi))
+        nout = n // 2 + 1
+        self.spec = torch.empty(2 * nout * b, dtype=torch.float32, device="cuda:0")
+        self.back = torch.empty(n * b, dtype=torch.float32, device="cuda:0")
+        self.scratch = torch.empty(1, dtype=torch.float32, device="cuda:0")
+        bp = F.BufParams1D(F.FFTLIB_FLOAT32, n * b)
+        self.bp = bp
+        self.pb_f = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_fft1dBatched_i32f_c32fc_o32fc"])
+        self.pb_i = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_ifft1dBatched_i32fc_c32fc_o32f"])
+        B = ctypes.byref(bp)
+        fa = (self.x.data_ptr(), B, self.w.data_ptr(), B, self.scratch.data_ptr(), B, self.sf.data_ptr(), B,
+              self.spec.data_ptr(), B, n, b, 0, self.pb_f)
+        ia = (self.spec.data_ptr(), B, self.wi.data_ptr(), B, self.scratch.data_ptr(), B, self.sfi.data_ptr(), B,
+              self.back.data_ptr(), B, n, b, 0, self.pb_i)
+        assert lib.FFTLIB_fft1dBatched_i32f_c32fc_o32fc_init(*fa) == 0
+        assert lib.FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_init(*ia) == 0
+        fwd, inv = lib.FFTLIB_fft1dBatched_i32f_c32fc_o32fc_kernel, lib.FFTLIB_ifft1dBatched_i32fc_c32fc_o32f_kernel
+        self.flops = 2 * 2.5 * n * math.log2(n) * b
+        self.bytes = 2.0 * (4.0 * n * b + 8.0 * nout * b)
+        self.dtype = "f32"
+        self.unit = "GFLOP/s"
+        self.config = {"workload": "R2C then C2R (FFTLIB_fft1dBatched_i32f_c32fc_o32fc + "
+                                   "FFTLIB_ifft1dBatched_i32fc_c32fc_o32f) N=4096 B=16384, outmode/inMode 0",
+                       "numPoints": n, "numChannels": b}
+        self.kernel_name = "r2c"
+
+        def step():
+            s1 = fwd(*fa)
+            s2 = inv(*ia)
+            if s1 or s2:
+                raise RuntimeError(f"status {s1},{s2}: {F.last_error()}")
+        self.step = step
+
+    def check_r2c_4096(self):
+        idx = np.arange(0, self.b, self.b // 64)
+        nout = self.n // 2 + 1
+        spec = self.spec.view(self.b, 2 * nout)[idx].cpu().numpy().view(np.complex64)
+        back = self.back.view(self.b, self.n)[idx].cpu().numpy()
+        x = self.host_x[idx].astype(np.float64)
+        ref = np.fft.rfft(x, axis=1)
+        return {"fwd_rel_l2_vs_fp64": float(np.linalg.norm(spec - ref) / np.linalg.norm(ref)),
+                "roundtrip_rel_l2": float(np.linalg.norm(back - x) / np.linalg.norm(x)),
+                "checked_channels": int(len(idx))}
+
+    def cpu_r2c_4096(self, seconds):
+        import oracle
+        cnt = 256
+        t = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            spec = oracle.r2c(self.host_x[:cnt], self.n, cnt, 0)
+            oracle.c2r(spec, self.n, cnt, 0)
+            t.append(time.perf_counter() - t0)
+        tm = float(np.median(t))
+        flops = 2 * 2.5 * self.n * math.log2(self.n) * cnt
+        return {"value": flops / tm / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+                "sample": f"oracle R2C+C2R _cn, N=4096, {cnt} of 16384 channels, median of 5 ({tm:.3f} s)"}
+
+
+def torch_equal(a, b):
+    return bool((a == b).all().item())
+
+
+def load_traffic(workload):
+    """Measured HBM bytes per launch from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the batch is described in "
+                                   "DESIGN.md but no multi-GPU path is built", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    import c7x_fftlib_amd as F
+    lib = F.load()
+    assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_GPU) == 0
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    lib.FFTLIB_GPU_setStream(ctypes.c_void_p(stream.cuda_stream))
+    if args.variant != "fp32":
+        sel = getattr(lib, "FFTLIB_GPU_setC2CVariant", None)
+        if sel is None:
+            raise SystemExit("variant selection not available in this library build")
+        code = {"fp32": 0, "bf16x3": 1, "fp16x2": 2}[args.variant]
+        assert sel(code) == 0, F.last_error()
+
+    wl = Workload(args, F, lib, torch)
+    lib.FFTLIB_GPU_setSynchronous(0)
+    for _ in range(args.warmup):
+        wl.step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        wl.step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lib.FFTLIB_GPU_setSynchronous(1)
+    ms = ev0.elapsed_time(ev1) / args.steps
+    t = ms / 1e3
+    achieved = wl.bytes / t / 1e9
+    value = wl.flops / t / 1e9 if wl.flops else achieved
+    traffic = load_traffic(args.workload)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": wl.unit, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": wl.dtype, "data": "synthetic", "config": wl.config,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "algorithmic_bytes_per_launch": wl.bytes, "kernel": wl.kernel_name,
+                     "timing": "HIP events on the launch stream, average over the timed steps"},
+        "wall_s_timed": round(wall, 4),
+    }
+    out["check"] = getattr(wl, "check_" + args.workload)()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = getattr(wl, "cpu_" + args.workload)(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
