@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "copy_503m"])
     ap.add_argument("--variant", default="fp32", help="fp32 C2C kernel variant (fp32, bf16x3, fp16x2)")
+    ap.add_argument("--generic", action="store_true",
+                    help="run the generic (runtime stage table) kernels instead of the length-specialised ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target seconds per CPU baseline run")
     return ap.parse_args()
@@ -93,7 +95,7 @@ class Workload:
         self.config = {"workload": "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc N=960 B=65536 (fp32 C2C)",
                        "numPoints": n, "numChannels": b, "variant": self.args.variant,
                        "input": "complex64 uniform[-1,1) Philox seed 960"}
-        self.kernel_name = "c2c"
+        self.kernel_name = "c2c_lds_kernel<0, true>" if self.args.generic else "c2c_fixed_kernel<960>"
         xp, wp, yp = self.x.data_ptr(), self.w.data_ptr(), self.y.data_ptr()
         bx, bw, pb = ctypes.byref(self.bp_x), ctypes.byref(self.bp_w), self.pblock
         fn = lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_kernel
@@ -311,12 +313,15 @@ def torch_equal(a, b):
     return bool((a == b).all().item())
 
 
-def load_traffic(workload):
-    """Measured HBM bytes per launch from the committed rocprofv3 summary."""
+def load_traffic(workload, kernel):
+    """Measured HBM bytes per launch from the committed rocprofv3 summary of
+    the same kernel (None when no summary covers it)."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     if os.path.exists(path):
         with open(path) as fh:
-            return json.load(fh)
+            t = json.load(fh)
+        if t.get("kernel") == kernel:
+            return t
     return None
 
 
@@ -334,6 +339,7 @@ def main():
     torch.cuda.init()
     stream = torch.cuda.current_stream()
     lib.FFTLIB_GPU_setStream(ctypes.c_void_p(stream.cuda_stream))
+    lib.FFTLIB_GPU_setSpecializedKernels(0 if args.generic else 1)
     if args.variant != "fp32":
         sel = getattr(lib, "FFTLIB_GPU_setC2CVariant", None)
         if sel is None:
@@ -359,11 +365,12 @@ def main():
     t = ms / 1e3
     achieved = wl.bytes / t / 1e9
     value = wl.flops / t / 1e9 if wl.flops else achieved
-    traffic = load_traffic(args.workload)
+    traffic = load_traffic(args.workload, wl.kernel_name)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": wl.unit, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": wl.dtype, "data": "synthetic", "config": wl.config,
+        "vs_baseline": None, "dtype": wl.dtype, "data": "synthetic",
+        "config": dict(wl.config, kernels="generic" if args.generic else "specialised"),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,

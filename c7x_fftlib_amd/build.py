@@ -19,6 +19,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 
 SOURCES = [
     "fft_kernels.hip",
+    "fft_fixed.hip",
     "fft_large.hip",
     "fft_mfma.hip",
     "fft_ragged.hip",

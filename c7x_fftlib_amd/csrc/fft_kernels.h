@@ -40,4 +40,12 @@ hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uin
 // Largest complex length the single-pass LDS kernels accept.
 uint32_t max_lds_points();
 
+// Lengths with a compile-time specialised complex kernel (fft_fixed.hip).
+bool fixed_c2c_supported(uint32_t n);
+
+// Batched complex FFT through the specialised kernel of length n (only for
+// fixed_c2c_supported(n)); same semantics as launch_c2c.
+hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t n, uint32_t numChannels, int mode,
+                            hipStream_t stream);
+
 }  // namespace fftlib

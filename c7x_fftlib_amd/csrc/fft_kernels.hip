@@ -19,144 +19,15 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_kernels.h"
+#include "fft_engine.h"
 
 namespace fftlib {
 namespace {
 
+using namespace engine;
+
 constexpr int      kThreads         = 256;
 constexpr uint32_t kTargetLdsPoints = 4096;  // complex data points resident per workgroup
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 sub2(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 w)
-{
-   return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
-}
-// -i * z
-__device__ __forceinline__ float2 mul_mi(float2 z) { return make_float2(z.y, -z.x); }
-
-// k / r and k % r for the digit radices 2, 3, 5 (constant divisors).
-__device__ __forceinline__ uint32_t div_digit(uint32_t k, uint32_t r)
-{
-   switch (r) {
-   case 2: return k >> 1;
-   case 3: return k / 3u;
-   case 5: return k / 5u;
-   default: return k / r;
-   }
-}
-
-// Position, in the in-place DIF result, of frequency k.
-__device__ __forceinline__ uint32_t dif_position(uint32_t k, const StageTable &st)
-{
-   if (st.pow2) {
-      return st.log2n ? (__brev(k) >> (32u - st.log2n)) : 0u;
-   }
-   uint32_t pos = 0;
-   for (uint32_t i = 0; i < st.nDigits; ++i) {
-      const uint32_t r  = st.digit[i];
-      const uint32_t qd = div_digit(k, r);
-      pos               = pos * r + (k - qd * r);
-      k                 = qd;
-   }
-   return pos;
-}
-
-// cos/sin of 2*pi/5, 4*pi/5 and 2*pi/3.
-constexpr float kC51 = 0.30901699437494745f;
-constexpr float kS51 = 0.95105651629515353f;
-constexpr float kC52 = -0.80901699437494734f;
-constexpr float kS52 = 0.58778525229247325f;
-constexpr float kS3  = 0.86602540378443871f;
-
-// ---------------------------------------------------------------------------
-// fp32 DIF stages in LDS
-// ---------------------------------------------------------------------------
-// Radix-5: X_m = sum_j x_j w5^{mj}, w5 = exp(-2 pi i / 5), then output m is
-// multiplied by its stage twiddle. Written with the symmetric sums
-// a1 = x1+x4, b1 = x1-x4, a2 = x2+x3, b2 = x2-x3.
-__device__ __forceinline__ void bfly5(float2 *b0, uint32_t q, const float2 *w, uint32_t p)
-{
-   const float2 x0 = b0[0], x1 = b0[q], x2 = b0[2 * q], x3 = b0[3 * q], x4 = b0[4 * q];
-   const float2 a1 = add2(x1, x4), b1 = sub2(x1, x4), a2 = add2(x2, x3), b2 = sub2(x2, x3);
-   const float2 c1 = make_float2(x0.x + kC51 * a1.x + kC52 * a2.x, x0.y + kC51 * a1.y + kC52 * a2.y);
-   const float2 c2 = make_float2(x0.x + kC52 * a1.x + kC51 * a2.x, x0.y + kC52 * a1.y + kC51 * a2.y);
-   const float2 s1 = make_float2(kS51 * b1.x + kS52 * b2.x, kS51 * b1.y + kS52 * b2.y);
-   const float2 s2 = make_float2(kS52 * b1.x - kS51 * b2.x, kS52 * b1.y - kS51 * b2.y);
-   b0[0]     = add2(add2(x0, a1), a2);
-   b0[q]     = cmul(add2(c1, mul_mi(s1)), w[p]);
-   b0[2 * q] = cmul(add2(c2, mul_mi(s2)), w[q + p]);
-   b0[3 * q] = cmul(sub2(c2, mul_mi(s2)), w[2 * q + p]);
-   b0[4 * q] = cmul(sub2(c1, mul_mi(s1)), w[3 * q + p]);
-}
-
-// Radix-3: X1,2 = x0 - a/2 -/+ i s3 b with a = x1+x2, b = x1-x2.
-__device__ __forceinline__ void bfly3(float2 *b0, uint32_t q, const float2 *w, uint32_t p)
-{
-   const float2 x0 = b0[0], x1 = b0[q], x2 = b0[2 * q];
-   const float2 a = add2(x1, x2), b = sub2(x1, x2);
-   const float2 c = make_float2(x0.x - 0.5f * a.x, x0.y - 0.5f * a.y);
-   const float2 s = mul_mi(make_float2(kS3 * b.x, kS3 * b.y));
-   b0[0]     = add2(x0, a);
-   b0[q]     = cmul(add2(c, s), w[p]);
-   b0[2 * q] = cmul(sub2(c, s), w[q + p]);
-}
-
-// Radix-4 with output twiddles; outputs 1 and 2 are stored swapped
-// (reference ..._cn.c:368-381).
-__device__ __forceinline__ void bfly4(float2 *b0, uint32_t q, const float2 *w, uint32_t p)
-{
-   const float2 x0 = b0[0], x1 = b0[q], x2 = b0[2 * q], x3 = b0[3 * q];
-   const float2 s02 = add2(x0, x2), d02 = sub2(x0, x2), s13 = add2(x1, x3), d13 = sub2(x1, x3);
-   b0[0]     = add2(s02, s13);
-   b0[q]     = cmul(sub2(s02, s13), w[q + p]);            // X2 -> slot 1
-   b0[2 * q] = cmul(add2(d02, mul_mi(d13)), w[p]);        // X1 -> slot 2
-   b0[3 * q] = cmul(sub2(d02, mul_mi(d13)), w[2 * q + p]); // X3
-}
-
-__device__ void dif_stages_f32(float2 *data, uint32_t nf, const float2 *twl, const StageTable &st)
-{
-   const uint32_t N = st.n;
-   uint32_t       M = N;
-   for (uint32_t s = 0; s < st.nStages; ++s) {
-      const uint32_t r     = st.radix[s];
-      const uint32_t q     = M / r;
-      const uint32_t nb    = N / r;
-      const uint32_t total = nb * nf;
-      const float2  *w     = twl + st.twOff[s];
-      for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
-         const uint32_t f = idx / nb;
-         const uint32_t k = idx - f * nb;
-         float2        *c = data + f * N;
-         if (r == 2) {
-            const float2 a = c[2 * k], b = c[2 * k + 1];
-            c[2 * k]       = add2(a, b);
-            c[2 * k + 1]   = sub2(a, b);
-         }
-         else {
-            const uint32_t dft = k / q;
-            const uint32_t p   = k - dft * q;
-            float2        *b0  = c + dft * M + p;
-            if (r == 4)
-               bfly4(b0, q, w, p);
-            else if (r == 5)
-               bfly5(b0, q, w, p);
-            else
-               bfly3(b0, q, w, p);
-         }
-      }
-      __syncthreads();
-      M = q;
-   }
-}
-
-__device__ __forceinline__ void load_twiddles(float2 *twl, const float2 *tw, uint32_t count)
-{
-   for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) twl[i] = tw[i];
-}
 
 // ---------------------------------------------------------------------------
 // fp32 kernels
@@ -171,26 +42,28 @@ __global__ void __launch_bounds__(kThreads) c2c_lds_kernel(const float2 *__restr
    const uint32_t first = blockIdx.x * F;
    const uint32_t nf    = min(F, numChannels - first);
    float2        *data  = smem;
-   const float2  *twl   = TWLDS ? smem + F * N : tw;
+   const float2  *twl   = TWLDS ? smem + lds_points(F, N) : tw;
    const size_t   base  = (size_t) first * N;
-   const float    scale = 1.0f / (float) N;
 
-   if (TWLDS) load_twiddles(smem + F * N, tw, st.twCount);
-   for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) {
-      float2 v = x[base + i];
-      if (MODE != C2C_FORWARD) v.y = -v.y;
-      if (MODE == C2C_INVERSE_PRESCALE) {
-         v.x = scale * v.x;
-         v.y = scale * v.y;
-      }
-      data[i] = v;
+   if (TWLDS) {
+      load_twiddles(smem + lds_points(F, N), tw, st.twCount);
+      __syncthreads();
    }
-   __syncthreads();
-   dif_stages_f32(data, nf, twl, st);
+   if (st.nPasses > 0) {
+      const GlobalIn gin{x + base, N, MODE, 1.0f / (float) N};
+      dif_passes_global_in(data, nf, twl, st, gin);
+   }
+   else {
+      for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) {
+         const GlobalIn gin{x + base, N, MODE, 1.0f / (float) N};
+         data[lds_idx(i)] = gin.load(0, i);
+      }
+      __syncthreads();
+   }
    for (uint32_t i = threadIdx.x; i < nf * N; i += blockDim.x) {
       const uint32_t f = i / N;
       const uint32_t k = i - f * N;
-      float2         v = data[f * N + dif_position(k, st)];
+      float2         v = data[lds_idx(f * N + dif_position(k, st))];
       if (MODE == C2C_INVERSE_PRESCALE) v.y = -v.y;
       if (MODE == C2C_INVERSE_POSTDIV) {
          v.x = v.x / (float) N;
@@ -217,18 +90,18 @@ __global__ void __launch_bounds__(kThreads) r2c_lds_kernel(const float2 *__restr
    const uint32_t first = blockIdx.x * F;
    const uint32_t nf    = min(F, numChannels - first);
    float2        *data  = smem;
-   const float2  *twl   = TWLDS ? smem + F * Nc : tw;
+   const float2  *twl   = TWLDS ? smem + lds_points(F, Nc) : tw;
 
-   if (TWLDS) load_twiddles(smem + F * Nc, tw, st.twCount);
-   for (uint32_t i = threadIdx.x; i < nf * Nc; i += blockDim.x) data[i] = x[(size_t) first * Nc + i];
+   if (TWLDS) load_twiddles(smem + lds_points(F, Nc), tw, st.twCount);
+   for (uint32_t i = threadIdx.x; i < nf * Nc; i += blockDim.x) data[lds_idx(i)] = x[(size_t) first * Nc + i];
    __syncthreads();
-   dif_stages_f32(data, nf, twl, st);
+   dif_passes_lds(data, nf, twl, st);
    for (uint32_t idx = threadIdx.x; idx < nf * Nc; idx += blockDim.x) {
       const uint32_t f   = idx / Nc;
       const uint32_t i   = idx - f * Nc;
-      const float2  *c   = data + f * Nc;
+      const uint32_t c   = f * Nc;
       float2        *out = y + (size_t) (first + f) * nOut;
-      const float2   zi  = c[dif_position(i, st)];
+      const float2   zi  = data[lds_idx(c + dif_position(i, st))];
       const float    ar = sf[2u * i], ai = sf[2u * i + 1u];
       const float    br = sf[2u * N - 2u * i - 2u], bi = sf[2u * N - 2u * i - 1u];
       float2         t1;
@@ -236,7 +109,7 @@ __global__ void __launch_bounds__(kThreads) r2c_lds_kernel(const float2 *__restr
          t1 = make_float2(zi.x * br + zi.y * bi, zi.x * br - zi.y * bi);
       }
       else {
-         const float2 zm = c[dif_position(Nc - i, st)];
+         const float2 zm = data[lds_idx(c + dif_position(Nc - i, st))];
          t1              = make_float2(zm.x * br + zm.y * bi, zm.x * bi - zm.y * br);
       }
       const float2 v = make_float2(zi.x * ar - zi.y * ai + t1.x, zi.x * ai + zi.y * ar + t1.y);
@@ -262,10 +135,10 @@ __global__ void __launch_bounds__(kThreads) c2r_lds_kernel(const float2 *__restr
    const uint32_t first    = blockIdx.x * F;
    const uint32_t nf       = min(F, numChannels - first);
    float2        *data     = smem;
-   const float2  *twl      = TWLDS ? smem + F * Nc : tw;
+   const float2  *twl      = TWLDS ? smem + lds_points(F, Nc) : tw;
    const float    scale    = 1.0f / (float) Nc;
 
-   if (TWLDS) load_twiddles(smem + F * Nc, tw, st.twCount);
+   if (TWLDS) load_twiddles(smem + lds_points(F, Nc), tw, st.twCount);
    for (uint32_t idx = threadIdx.x; idx < nf * Nc; idx += blockDim.x) {
       const uint32_t f  = idx / Nc;
       const uint32_t i  = idx - f * Nc;
@@ -288,14 +161,14 @@ __global__ void __launch_bounds__(kThreads) c2r_lds_kernel(const float2 *__restr
          z.x = scale * z.x;
          z.y = scale * z.y;
       }
-      data[idx] = z;
+      data[lds_idx(idx)] = z;
    }
    __syncthreads();
-   dif_stages_f32(data, nf, twl, st);
+   dif_passes_lds(data, nf, twl, st);
    for (uint32_t idx = threadIdx.x; idx < nf * Nc; idx += blockDim.x) {
       const uint32_t f = idx / Nc;
       const uint32_t n = idx - f * Nc;
-      float2         v = data[f * Nc + dif_position(n, st)];
+      float2         v = data[lds_idx(f * Nc + dif_position(n, st))];
       if (POSTDIV) {
          v.x = v.x / (float) Nc;
          v.y = -(v.y / (float) Nc);
@@ -460,19 +333,21 @@ struct Geometry {
    bool     twInLds;
 };
 
-bool pick_geometry(uint32_t n, uint32_t elemBytes, uint32_t twCount, uint32_t numChannels, Geometry *g)
+bool pick_geometry(uint32_t n, uint32_t elemBytes, uint32_t twCount, uint32_t numChannels, Geometry *g,
+                   bool padded = true)
 {
+   auto slots = [padded](uint32_t f, uint32_t nn) { return padded ? lds_points(f, nn) : f * nn; };
    const size_t limit = max_lds_bytes();
    uint32_t     F     = n >= kTargetLdsPoints ? 1u : kTargetLdsPoints / n;
    if (F > numChannels) F = numChannels;
    if (F == 0) F = 1;
    for (int withTw = 1; withTw >= 0; --withTw) {
       uint32_t f = F;
-      while (f > 0 && ((size_t) f * n + (withTw ? twCount : 0u)) * elemBytes > limit) f >>= 1;
+      while (f > 0 && ((size_t) slots(f, n) + (withTw ? twCount : 0u)) * elemBytes > limit) f >>= 1;
       if (f > 0) {
          g->F        = f;
          g->twInLds  = withTw != 0;
-         g->ldsBytes = ((size_t) f * n + (withTw ? twCount : 0u)) * elemBytes;
+         g->ldsBytes = ((size_t) slots(f, n) + (withTw ? twCount : 0u)) * elemBytes;
          return true;
       }
    }
@@ -551,7 +426,7 @@ hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uin
 {
    if (numChannels == 0) return hipSuccess;
    Geometry g;
-   if (!pick_geometry(st.n, 4u, st.twCount, numChannels, &g) || !g.twInLds) return hipErrorInvalidValue;
+   if (!pick_geometry(st.n, 4u, st.twCount, numChannels, &g, false) || !g.twInLds) return hipErrorInvalidValue;
    return launch(i16_lds_kernel, g, numChannels, stream, (const uint32_t *) x, (uint32_t *) y,
                  (const uint32_t *) tw, shift, st, numChannels, g.F);
 }

@@ -7,13 +7,17 @@
 // each stage's twiddle block lives in the caller's twiddle table, plus the
 // digit sequence that maps a frequency index to its position in the in-place
 // result (..._cn.c:409-491).
+//
+// For the GPU the stages are grouped into passes of one or two consecutive
+// stages; a work item keeps all r1*r2 points a pass touches in registers, so
+// one LDS round trip serves two stages.
 #pragma once
 
 #include <stdint.h>
 
 namespace fftlib {
 
-constexpr int kMaxStages = 24;
+constexpr int kMaxStages = 20;  // lengths up to 2^20 need at most 20 digits
 
 struct StageTable {
    uint32_t n;        // complex FFT length
@@ -22,9 +26,13 @@ struct StageTable {
    uint32_t log2n;    // log2(n) when pow2
    uint32_t nDigits;  // digits of the mixed-radix reversal (when !pow2)
    uint32_t twCount;  // complex twiddles the stages read (table prefix length)
-   uint8_t  radix[kMaxStages];  // 5, 3, 4 or 2 per stage, in execution order
-   uint8_t  digit[kMaxStages];  // reversal digit radices, least significant first
-   uint32_t twOff[kMaxStages];  // complex offset of each stage's twiddle block
+   uint32_t nPasses;  // register passes
+   uint8_t  radix[kMaxStages];      // 5, 3, 4 or 2 per stage, in execution order
+   uint8_t  digit[kMaxStages];      // reversal digit radices, least significant first
+   uint8_t  passFirst[kMaxStages];  // first stage of each pass
+   uint8_t  passLen[kMaxStages];    // stages in each pass (1 or 2)
+   uint32_t twOff[kMaxStages];      // complex offset of each stage's twiddle block
+   uint32_t stageM[kMaxStages];     // sub-DFT length entering each stage
 };
 
 // Builds the stage table of a 2/3/5-smooth length; returns false otherwise.

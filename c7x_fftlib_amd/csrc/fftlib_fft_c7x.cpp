@@ -9,7 +9,7 @@
 // as scratch). The fused real-transform kernels do not write pXFFT.
 #include <string.h>
 
-#include "../../include/fftlib.h"
+#include "fftlib.h"
 #include "fft_kernels.h"
 #include "fftlib_runtime.h"
 
@@ -74,6 +74,9 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    int mode = C2C_FORWARD;
    if (p.kernel == K_IC2C_BATCHED) mode = C2C_INVERSE_PRESCALE;
    if (p.kernel == K_IC2C_SINGLE) mode = C2C_INVERSE_POSTDIV;
+   if (specialized_kernels() && fixed_c2c_supported(p.st.n))
+      return launch_c2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st.n, p.numChannels,
+                              mode, s);
    return launch_c2c((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st, p.numChannels, mode, s);
 }
 

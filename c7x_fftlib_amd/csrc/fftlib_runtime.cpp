@@ -9,7 +9,7 @@
 #include <atomic>
 #include <mutex>
 
-#include "../../include/fftlib_gpu.h"
+#include "fftlib_gpu.h"
 
 namespace fftlib {
 namespace {
@@ -17,6 +17,7 @@ namespace {
 std::atomic<int>         g_style{FFTLIB_FUNCTION_OPTIMIZED};
 std::atomic<hipStream_t> g_stream{nullptr};
 std::atomic<int>         g_sync{1};
+std::atomic<int>         g_specialized{1};
 thread_local char        t_lastError[256] = "";
 
 // Device staging for host-memory callers: one grow-only buffer per role,
@@ -53,6 +54,7 @@ hipError_t ensure(int slot, size_t bytes)
 FFTLIB_FUNCTION_STYLE current_style() { return (FFTLIB_FUNCTION_STYLE) g_style.load(); }
 hipStream_t           current_stream() { return g_stream.load(); }
 bool                  synchronous() { return g_sync.load() != 0; }
+bool                  specialized_kernels() { return g_specialized.load() != 0; }
 
 void set_last_error(const char *what, hipError_t err)
 {
@@ -106,6 +108,8 @@ FFTLIB_STATUS run_on_gpu(const Plan &plan, const CallBuffers &b, LaunchFn launch
 // twiddle block of a radix-r stage with sub-length M holds (r-1)*(M/r) complex
 // values (..._cn.c:305, :335, :385). The output permutation uses the digits
 // [5..., 3..., 2,2 per radix-4, 2] (..._cn.c:430-458) or bit reversal.
+// Consecutive stages are then paired into register passes of at most 25
+// points per work item.
 bool build_stage_table(uint32_t n, StageTable *st)
 {
    memset(st, 0, sizeof *st);
@@ -117,6 +121,7 @@ bool build_stage_table(uint32_t n, StageTable *st)
       const uint32_t q          = M / r;
       st->radix[st->nStages]    = (uint8_t) r;
       st->twOff[st->nStages]    = off;
+      st->stageM[st->nStages]   = M;
       st->nStages++;
       if (r != 2) off += (r - 1) * q;
       M = q;
@@ -154,6 +159,13 @@ bool build_stage_table(uint32_t n, StageTable *st)
       while ((1u << l) < n) ++l;
       st->log2n = l;
    }
+   for (uint32_t s = 0; s < st->nStages;) {
+      const bool pair = s + 1 < st->nStages && (uint32_t) st->radix[s] * st->radix[s + 1] <= 25u;
+      st->passFirst[st->nPasses] = (uint8_t) s;
+      st->passLen[st->nPasses]   = pair ? 2 : 1;
+      st->nPasses++;
+      s += pair ? 2u : 1u;
+   }
    return true;
 }
 
@@ -184,6 +196,12 @@ void *FFTLIB_GPU_getStream(void) { return (void *) current_stream(); }
 FFTLIB_STATUS FFTLIB_GPU_setSynchronous(int32_t sync)
 {
    g_sync.store(sync != 0);
+   return FFTLIB_SUCCESS;
+}
+
+FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable)
+{
+   g_specialized.store(enable != 0);
    return FFTLIB_SUCCESS;
 }
 

@@ -7,7 +7,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
-#include "../../include/FFTLIB_types.h"
+#include "FFTLIB_types.h"
 #include "fft_plan.h"
 
 namespace fftlib {
@@ -45,6 +45,7 @@ FFTLIB_STATUS run_on_gpu(const Plan &plan, const CallBuffers &b, LaunchFn launch
 FFTLIB_FUNCTION_STYLE current_style();
 hipStream_t           current_stream();
 bool                  synchronous();
+bool                  specialized_kernels();
 void                  set_last_error(const char *what, hipError_t err);
 
 }  // namespace fftlib

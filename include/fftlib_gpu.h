@@ -33,6 +33,12 @@ void *FFTLIB_GPU_getStream(void);
  * always complete before returning. */
 FFTLIB_STATUS FFTLIB_GPU_setSynchronous(int32_t synchronous);
 
+/* 1 (default): complex FFT lengths with a compile-time specialised kernel use
+ * it; 0: every length runs the generic (runtime stage table) kernels. Both
+ * give results within fp32 rounding of each other; the switch exists for
+ * tuning and for testing the generic kernels at every length. */
+FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable);
+
 /* Waits for all work the library enqueued on its stream. */
 FFTLIB_STATUS FFTLIB_GPU_synchronize(void);
 

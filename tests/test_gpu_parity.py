@@ -44,6 +44,42 @@ def test_c2c_batched(n, inverse, style):
     assert np.max(np.abs(y - ref)) <= max(tol, 1e-6 * np.max(np.abs(ref)))
 
 
+# Lengths served by the compile-time specialised complex kernels; the batch
+# counts leave a partly filled last workgroup.
+FIXED_SIZES = [16, 32, 48, 64, 96, 128, 192, 240, 256, 480, 512, 960, 1000, 1024, 1200, 1536, 1920, 2000, 2048,
+               3200, 3840, 4000, 4096, 6400, 8192, 16384]
+
+
+@pytest.fixture
+def generic_kernels():
+    lib = G.lib()
+    assert lib.FFTLIB_GPU_setSpecializedKernels(0) == 0
+    yield
+    assert lib.FFTLIB_GPU_setSpecializedKernels(1) == 0
+
+
+@pytest.mark.parametrize("kind", ["fft", "ifft", "ifft_single"])
+@pytest.mark.parametrize("n", FIXED_SIZES)
+def test_c2c_specialised_lengths(n, kind):
+    single = kind == "ifft_single"
+    b = 1 if single else (3 if n >= 8192 else (5 if n >= 2048 else 37))
+    x = cplx(rng(n * 13 + len(kind)), b, n, 1.0)
+    s, y = G.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert s == 0, G.F.last_error()
+    ref = oracle.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert G.rel_l2(y, ref) <= REL_TOL
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("n", [16, 48, 960, 1024, 4096])
+def test_c2c_generic_kernels_at_specialised_lengths(n, inverse, generic_kernels):
+    b = 9
+    x = cplx(rng(n * 17 + inverse), b, n, 1.0)
+    s, y = G.c2c(x, n, b, inverse=inverse)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.c2c(x, n, b, inverse=inverse)) <= REL_TOL
+
+
 @pytest.mark.parametrize("inverse", [False, True])
 @pytest.mark.parametrize("n", [16, 64, 128, 256, 512, 1024, 2048, 4096, 30, 60, 120, 240, 480, 960, 1920, 3840,
                                15625])
