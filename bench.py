@@ -274,7 +274,8 @@ class Workload:
         self.config = {"workload": "R2C then C2R (FFTLIB_fft1dBatched_i32f_c32fc_o32fc + "
                                    "FFTLIB_ifft1dBatched_i32fc_c32fc_o32f) N=4096 B=16384, outmode/inMode 0",
                        "numPoints": n, "numChannels": b}
-        self.kernel_name = "r2c"
+        self.kernel_name = ("r2c_lds_kernel + c2r_lds_kernel" if self.args.generic
+                            else "fixed_kernel<2048, R2C> + fixed_kernel<2048, C2R>")
 
         def step():
             s1 = fwd(*fa)

@@ -82,11 +82,14 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f)
 // Radix-R butterfly on v[o + S*m], m < R: the R-point DFT, outputs m >= 1
 // multiplied by their stage twiddles w[(m-1)*wq + p], results stored in the
 // reference's in-place slot order (radix-4 outputs 1 and 2 swapped,
-// ..._cn.c:368-381).
-template <int R>
+// ..._cn.c:368-381). TW = false is the last stage of a transform (wq == 1,
+// p == 0), whose twiddles are the table's W^0 = (1, -0) entries: the
+// products are skipped (they change at most the sign of a zero).
+template <int R, bool TW = true>
 __device__ __forceinline__ void butterfly(float2 *v, const int o, const int S, const float2 *w, uint32_t wq,
                                           uint32_t p)
 {
+   auto tw = [&](float2 a, uint32_t i) { return TW ? cmul(a, w[i]) : a; };
    if constexpr (R == 2) {
       const float2 a = v[o], b = v[o + S];
       v[o]           = add2(a, b);
@@ -98,16 +101,16 @@ __device__ __forceinline__ void butterfly(float2 *v, const int o, const int S, c
       const float2 c = make_float2(x0.x - 0.5f * a.x, x0.y - 0.5f * a.y);
       const float2 s = mul_mi(make_float2(kS3 * b.x, kS3 * b.y));
       v[o]         = add2(x0, a);
-      v[o + S]     = cmul(add2(c, s), w[p]);
-      v[o + 2 * S] = cmul(sub2(c, s), w[wq + p]);
+      v[o + S]     = tw(add2(c, s), p);
+      v[o + 2 * S] = tw(sub2(c, s), wq + p);
    }
    else if constexpr (R == 4) {
       const float2 x0 = v[o], x1 = v[o + S], x2 = v[o + 2 * S], x3 = v[o + 3 * S];
       const float2 s02 = add2(x0, x2), d02 = sub2(x0, x2), s13 = add2(x1, x3), d13 = sub2(x1, x3);
       v[o]         = add2(s02, s13);
-      v[o + S]     = cmul(sub2(s02, s13), w[wq + p]);             // X2 -> slot 1
-      v[o + 2 * S] = cmul(add2(d02, mul_mi(d13)), w[p]);          // X1 -> slot 2
-      v[o + 3 * S] = cmul(sub2(d02, mul_mi(d13)), w[2 * wq + p]); // X3
+      v[o + S]     = tw(sub2(s02, s13), wq + p);             // X2 -> slot 1
+      v[o + 2 * S] = tw(add2(d02, mul_mi(d13)), p);          // X1 -> slot 2
+      v[o + 3 * S] = tw(sub2(d02, mul_mi(d13)), 2 * wq + p); // X3
    }
    else {
       static_assert(R == 5, "radix 2, 3, 4 or 5");
@@ -118,10 +121,10 @@ __device__ __forceinline__ void butterfly(float2 *v, const int o, const int S, c
       const float2 s1 = make_float2(kS51 * b1.x + kS52 * b2.x, kS51 * b1.y + kS52 * b2.y);
       const float2 s2 = make_float2(kS52 * b1.x - kS51 * b2.x, kS52 * b1.y - kS51 * b2.y);
       v[o]         = add2(add2(x0, a1), a2);
-      v[o + S]     = cmul(add2(c1, mul_mi(s1)), w[p]);
-      v[o + 2 * S] = cmul(add2(c2, mul_mi(s2)), w[wq + p]);
-      v[o + 3 * S] = cmul(sub2(c2, mul_mi(s2)), w[2 * wq + p]);
-      v[o + 4 * S] = cmul(sub2(c1, mul_mi(s1)), w[3 * wq + p]);
+      v[o + S]     = tw(add2(c1, mul_mi(s1)), p);
+      v[o + 2 * S] = tw(add2(c2, mul_mi(s2)), wq + p);
+      v[o + 3 * S] = tw(sub2(c2, mul_mi(s2)), 2 * wq + p);
+      v[o + 4 * S] = tw(sub2(c1, mul_mi(s1)), 3 * wq + p);
    }
 }
 

@@ -48,4 +48,13 @@ bool fixed_c2c_supported(uint32_t n);
 hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t n, uint32_t numChannels, int mode,
                             hipStream_t stream);
 
+// Inner complex lengths nc (= N/2) with specialised real-transform kernels.
+bool fixed_real_supported(uint32_t nc);
+
+// Specialised counterparts of launch_r2c / launch_c2r for nc = st.n.
+hipError_t launch_r2c_fixed(const float *x, float *y, const float *tw, const float *sf, uint32_t nc,
+                            uint32_t numChannels, uint32_t outmode, hipStream_t stream);
+hipError_t launch_c2r_fixed(const float *x, float *y, const float *tw, const float *sf, uint32_t nc,
+                            uint32_t numChannels, uint32_t inMode, int postdiv, hipStream_t stream);
+
 }  // namespace fftlib

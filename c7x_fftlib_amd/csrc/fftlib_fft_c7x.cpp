@@ -82,12 +82,18 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 
 hipError_t run_r2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
+   if (specialized_kernels() && fixed_real_supported(p.st.n))
+      return launch_r2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0,
+                              p.st.n, p.numChannels, p.mode, s);
    return launch_r2c((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0, p.st,
                      p.numChannels, p.mode, s);
 }
 
 hipError_t run_c2r(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
+   if (specialized_kernels() && fixed_real_supported(p.st.n))
+      return launch_c2r_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0,
+                              p.st.n, p.numChannels, p.mode, p.kernel == K_C2R_SINGLE ? 1 : 0, s);
    return launch_c2r((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0, p.st,
                      p.numChannels, p.mode, p.kernel == K_C2R_SINGLE ? 1 : 0, s);
 }

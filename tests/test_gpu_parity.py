@@ -167,6 +167,37 @@ def test_c2r_batched(n, b, inmode):
     assert np.max(np.abs(y - ref)) <= 1e-3
 
 
+# Real lengths N whose inner complex length N/2 has specialised kernels.
+REAL_FIXED_SIZES = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 960, 1920]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n", REAL_FIXED_SIZES)
+def test_real_specialised_lengths(n, mode):
+    b = 3 if n >= 8192 else 37
+    x = rng(n * 5 + mode).uniform(-1, 1, (b, n)).astype(np.float32)
+    s, y = G.r2c(x, n, b, outmode=mode)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.r2c(x, n, b, outmode=mode)) <= REL_TOL
+    bins = n // 2 + 1 if mode == 0 else n
+    spec = cplx(rng(n * 7 + mode), b, bins, 1.0)
+    s, z = G.c2r(spec, n, b, inmode=mode)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(z, oracle.c2r(spec, n, b, inmode=mode)) <= REL_TOL
+
+
+@pytest.mark.parametrize("n", [64, 960, 4096])
+def test_real_generic_kernels_at_specialised_lengths(n, generic_kernels):
+    b = 5
+    x = rng(n * 11).uniform(-1, 1, (b, n)).astype(np.float32)
+    s, y = G.r2c(x, n, b, outmode=0)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.r2c(x, n, b, outmode=0)) <= REL_TOL
+    s, z = G.c2r(y, n, b, inmode=0)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(z, oracle.c2r(y, n, b, inmode=0)) <= REL_TOL
+
+
 @pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096, 8192, 10, 50, 250, 1250])
 def test_real_single(n):
     x = rng(n).uniform(-1, 1, (1, n)).astype(np.float32)
