@@ -340,52 +340,81 @@ __device__ __forceinline__ void pass0_compute(float2 (&v)[Pass0<N>::iters][Pass0
       pass0_compute_one<N>(v[it], data, t + (uint32_t) (it * Pass0<N>::P.T), tw, sc);
 }
 
-// Output value k of the complex FFT in LDS after the inverse adjustments.
-__device__ __forceinline__ float2 out_adjust(float2 v, int mode, float n)
-{
-   if (mode == C2C_INVERSE_PRESCALE) v.y = -v.y;
-   if (mode == C2C_INVERSE_POSTDIV) {
-      v.x = v.x / n;
-      v.y = -(v.y / n);
-   }
-   return v;
-}
+// Output k of the complex FFT in LDS after the single-channel inverse's
+// division (the other modes are a sign factor applied by the caller).
+__device__ __forceinline__ float2 out_postdiv(float2 v, float n) { return make_float2(v.x / n, -(v.y / n)); }
 
-// Writes transform fl of the group (channel `ch`) from LDS: work item t
-// handles outputs t, t + T, ... in chunks of 8 (all LDS reads of a chunk are
-// issued before its stores). C2C and C2R write the N complex values in
-// natural order; R2C applies the split Y[i] = Z[i] A[i] + (Z[N-i] mirrored) B[i]
-// for i < N, the Nyquist bin Y[N] = Z0re - Z0im, and for outmode 1 the
-// conjugate upper half Y[2N-i] = conj(Y[i]).
+// LDS slots the store stage of work item t reads: outputs k = t + T*it of
+// its transform (and, for R2C, the mirrored N - k). They are the same for
+// every group of a persistent workgroup, so they are computed once when they
+// fit in a few registers (at most 16 outputs per work item).
 template <int N, int KIND>
-__device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint32_t t, const FixedArgs &a)
+struct StoreMap {
+   static constexpr FixedPlan P      = Fixed<N>::P;
+   static constexpr int       iters  = (N + P.T - 1) / P.T;
+   static constexpr bool      cached = iters <= 16;
+   static constexpr int       slots  = cached ? iters : 1;
+   uint32_t                   off[slots];
+   uint32_t                   mir[KIND == KIND_R2C ? slots : 1];
+
+   __device__ __forceinline__ void init(uint32_t t)
+   {
+      if constexpr (cached) {
+#pragma unroll
+         for (int it = 0; it < iters; ++it) {
+            const uint32_t k = min(t + (uint32_t) (it * P.T), (uint32_t) N - 1u);
+            off[it]          = fslot(fixed_position<N>(k));
+            if constexpr (KIND == KIND_R2C) mir[it] = fslot(fixed_position<N>(k == 0u ? 0u : (uint32_t) N - k));
+         }
+      }
+   }
+   __device__ __forceinline__ uint32_t at(int it, uint32_t t) const
+   {
+      if constexpr (cached) return off[it];
+      else return fslot(fixed_position<N>(min(t + (uint32_t) (it * P.T), (uint32_t) N - 1u)));
+   }
+   __device__ __forceinline__ uint32_t mirror_at(int it, uint32_t t) const
+   {
+      if constexpr (cached && KIND == KIND_R2C) return mir[it];
+      else {
+         const uint32_t k = min(t + (uint32_t) (it * P.T), (uint32_t) N - 1u);
+         return fslot(fixed_position<N>(k == 0u ? 0u : (uint32_t) N - k));
+      }
+   }
+};
+
+// Writes transform `ch` from its LDS slots `src`: work item t handles outputs
+// t, t + T, ... in chunks of up to 16 (8 for R2C; all LDS reads of a chunk
+// are issued before its stores). C2C and C2R write the N complex values in natural
+// order; R2C applies the split Y[i] = Z[i] A[i] + (Z[N-i] mirrored) B[i] for
+// i < N, the Nyquist bin Y[N] = Z0re - Z0im, and for outmode 1 the conjugate
+// upper half Y[2N-i] = conj(Y[i]).
+template <int N, int KIND>
+__device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint32_t t, const FixedArgs &a,
+                                            const StoreMap<N, KIND> &map)
 {
    constexpr FixedPlan P     = Fixed<N>::P;
    constexpr int       T     = P.T;
-   constexpr int       iters = (N + T - 1) / T;
-   constexpr int       CH    = iters < 8 ? iters : 8;
+   constexpr int       iters = StoreMap<N, KIND>::iters;
+   constexpr int       MAXC  = KIND == KIND_R2C ? 8 : 16;
+   constexpr int       CH    = iters < MAXC ? iters : MAXC;
+   constexpr bool      EXACT = N % T == 0 && iters % CH == 0;
    float2             *out   = a.y + (size_t) ch * a.outStride;
+   const float         sy    = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
+   const bool          pdiv  = a.mode == C2C_INVERSE_POSTDIV;
 #pragma unroll 1
    for (int c0 = 0; c0 < iters; c0 += CH) {
       float2 v[CH];
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-         uint32_t k = t + (uint32_t) ((c0 + c) * T);
-         if (N % T != 0 || iters % CH != 0) k = min(k, (uint32_t) N - 1u);
-         v[c] = src[fslot(fixed_position<N>(k))];
-      }
+      for (int c = 0; c < CH; ++c) v[c] = src[map.at(c0 + c, t)];
       if constexpr (KIND == KIND_R2C) {
          float2 vm[CH];
 #pragma unroll
-         for (int c = 0; c < CH; ++c) {
-            uint32_t k = t + (uint32_t) ((c0 + c) * T);
-            if (N % T != 0 || iters % CH != 0) k = min(k, (uint32_t) N - 1u);
-            vm[c] = src[fslot(fixed_position<N>(k == 0u ? 0u : (uint32_t) N - k))];
-         }
+         for (int c = 0; c < CH; ++c) vm[c] = src[map.mirror_at(c0 + c, t)];
 #pragma unroll
          for (int c = 0; c < CH; ++c) {
             const uint32_t i = t + (uint32_t) ((c0 + c) * T);
-            if ((N % T != 0 || iters % CH != 0) && (c0 + c >= iters || i >= (uint32_t) N)) break;
+            if (!EXACT && (c0 + c >= iters || i >= (uint32_t) N)) break;
             const float2 zi = v[c], zm = vm[c];
             const float2 A  = *(const float2 *) (a.sf + 2u * i);
             const float2 B  = *(const float2 *) (a.sf + 4u * N - 2u * i - 2u);
@@ -398,15 +427,30 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
             if (i == 0u) out[N] = make_float2(zi.x - zi.y, 0.0f);
          }
       }
+      else if (pdiv) {
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t k = t + (uint32_t) ((c0 + c) * T);
+            if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
+            out[k] = out_postdiv(v[c], (float) N);
+         }
+      }
       else {
 #pragma unroll
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
-            if ((N % T != 0 || iters % CH != 0) && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k] = out_adjust(v[c], a.mode, (float) N);
+            if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
+            out[k] = make_float2(v[c].x, sy * v[c].y);
          }
       }
    }
+}
+
+constexpr int largest_divisor_upto(int n, int cap)
+{
+   int c = n < cap ? n : cap;
+   while (n % c != 0) --c;
+   return c;
 }
 
 template <int N>
@@ -455,6 +499,8 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
    const InScale  sc   = in_scale(a.mode, 1.0f / (float) N);
    float2        *data = smem + fl * STRIDE;
    const float2  *twp  = fixed_tw_lds<N>() ? (const float2 *) (smem + FPB * STRIDE) : a.tw;
+   StoreMap<N, KIND> map;
+   map.init(t);
 
    if constexpr (fixed_prefetch<N, KIND>()) {
       using Q = Pass0<N>;
@@ -473,7 +519,37 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          if (gn < a.numGroups) pass0_load<N, KIND>(v, gn * FPB, fl, min((uint32_t) FPB, a.numChannels - gn * FPB), t, a);
          __syncthreads();
          fixed_passes_from<N, 1>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         __syncthreads();
+      }
+   }
+   else if constexpr (KIND == KIND_C2R && N % T == 0 && N >= 256) {
+      // C2R: work item t computes the pre-split points t + T*it of its
+      // transform from coalesced (and mirrored) reads, 8 at a time, into LDS;
+      // then every pass runs from LDS.
+      if constexpr (fixed_tw_lds<N>()) {
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+      }
+      constexpr int iters = N / T;
+      constexpr int CH    = largest_divisor_upto(iters, 8);
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+         const uint32_t first = g * FPB;
+         const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+         if (fl < nf) {
+            const float2 *in = a.x + (size_t) (first + fl) * a.inStride;
+#pragma unroll 1
+            for (int c0 = 0; c0 < iters; c0 += CH) {
+               float2 z[CH];
+#pragma unroll
+               for (int c = 0; c < CH; ++c) z[c] = fixed_input<N, KIND>(in, t + (uint32_t) ((c0 + c) * T), a);
+#pragma unroll
+               for (int c = 0; c < CH; ++c)
+                  data[fslot(t + (uint32_t) ((c0 + c) * T))] = make_float2(sc.sx * z[c].x, sc.sy * z[c].y);
+            }
+         }
+         __syncthreads();
+         fixed_passes_from<N, 0>(data, t, twp);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
          __syncthreads();
       }
    }
@@ -494,7 +570,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          }
          __syncthreads();
          fixed_passes_from<N, 1>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
          __syncthreads();
       }
    }
@@ -513,7 +589,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          }
          __syncthreads();
          fixed_passes_from<N, 0>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
          __syncthreads();
       }
    }
