@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "copy_503m"])
-    ap.add_argument("--variant", default="fp32", help="fp32 C2C kernel variant (fp32, bf16x3, fp16x2)")
+    ap.add_argument("--variant", default="fp32", choices=["fp32", "bf16x3", "fp16x2", "bf16x2"],
+                    help="C2C arithmetic: fp32 mixed radix, or MFMA DFT-as-GEMM with split bf16/fp16 operands")
     ap.add_argument("--generic", action="store_true",
                     help="run the generic (runtime stage table) kernels instead of the length-specialised ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -95,7 +96,12 @@ class Workload:
         self.config = {"workload": "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc N=960 B=65536 (fp32 C2C)",
                        "numPoints": n, "numChannels": b, "variant": self.args.variant,
                        "input": "complex64 uniform[-1,1) Philox seed 960"}
-        self.kernel_name = "c2c_lds_kernel<0, true>" if self.args.generic else "c2c_fixed_kernel<960>"
+        if self.args.variant != "fp32":
+            self.kernel_name = "c2c960_mfma_kernel<%s>" % self.args.variant
+        elif self.args.generic:
+            self.kernel_name = "c2c_lds_kernel<0, true>"
+        else:
+            self.kernel_name = "fixed_kernel<960, C2C>"
         xp, wp, yp = self.x.data_ptr(), self.w.data_ptr(), self.y.data_ptr()
         bx, bw, pb = ctypes.byref(self.bp_x), ctypes.byref(self.bp_w), self.pblock
         fn = lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_kernel
@@ -345,7 +351,7 @@ def main():
         sel = getattr(lib, "FFTLIB_GPU_setC2CVariant", None)
         if sel is None:
             raise SystemExit("variant selection not available in this library build")
-        code = {"fp32": 0, "bf16x3": 1, "fp16x2": 2}[args.variant]
+        code = {"fp32": 0, "bf16x3": 1, "fp16x2": 2, "bf16x2": 3}[args.variant]
         assert sel(code) == 0, F.last_error()
 
     wl = Workload(args, F, lib, torch)

@@ -94,6 +94,8 @@ _EXTRA = {
     "FFTLIB_GPU_getStream": (_P, []),
     "FFTLIB_GPU_setSynchronous": (_S, [ctypes.c_int32]),
     "FFTLIB_GPU_setSpecializedKernels": (_S, [ctypes.c_int32]),
+    "FFTLIB_GPU_setC2CVariant": (_S, [ctypes.c_int32]),
+    "FFTLIB_GPU_getC2CVariant": (ctypes.c_int32, []),
     "FFTLIB_GPU_synchronize": (_S, []),
     "FFTLIB_GPU_getLastError": (ctypes.c_char_p, []),
     "FFTLIB_GPU_version": (ctypes.c_char_p, []),

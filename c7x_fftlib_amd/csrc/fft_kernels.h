@@ -17,6 +17,23 @@ enum C2CMode : int {
    C2C_INVERSE_POSTDIV  = 2,  // single-channel IFFT: conj up front, / (float)N and conj on store
 };
 
+// Arithmetic of the batched complex FFT (FFTLIB_GPU_setC2CVariant).
+enum C2CVariant : int {
+   C2C_VARIANT_FP32   = 0,  // fp32 mixed-radix DIF (reference algorithm)
+   C2C_VARIANT_BF16X3 = 1,  // DFT-as-GEMM on MFMA, bf16 hi+mid+lo operands, 6 products
+   C2C_VARIANT_FP16X2 = 2,  // DFT-as-GEMM on MFMA, fp16 hi+lo operands, 3 products
+   C2C_VARIANT_BF16X2 = 3,  // DFT-as-GEMM on MFMA, bf16 hi+lo operands, 3 products
+};
+
+// Lengths with an MFMA DFT-as-GEMM kernel (fft_mfma.hip).
+bool mfma_c2c_supported(uint32_t n);
+
+// Batched complex FFT of n points through the MFMA kernel of `variant`
+// (one of the MFMA variants above); same modes as launch_c2c. The
+// caller's twiddle table is not read.
+hipError_t launch_c2c_mfma(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, int variant,
+                           hipStream_t stream);
+
 // Batched complex FFT of st.n points per channel, channels contiguous.
 hipError_t launch_c2c(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
                       int mode, hipStream_t stream);

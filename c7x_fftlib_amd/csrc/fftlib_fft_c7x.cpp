@@ -74,6 +74,9 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    int mode = C2C_FORWARD;
    if (p.kernel == K_IC2C_BATCHED) mode = C2C_INVERSE_PRESCALE;
    if (p.kernel == K_IC2C_SINGLE) mode = C2C_INVERSE_POSTDIV;
+   const int variant = c2c_variant();
+   if (variant != C2C_VARIANT_FP32 && mfma_c2c_supported(p.st.n))
+      return launch_c2c_mfma((const float *) d.in, (float *) d.out, p.st.n, p.numChannels, mode, variant, s);
    if (specialized_kernels() && fixed_c2c_supported(p.st.n))
       return launch_c2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st.n, p.numChannels,
                               mode, s);

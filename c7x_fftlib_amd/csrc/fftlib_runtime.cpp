@@ -1,6 +1,7 @@
 // GPU runtime of the C-ABI layer, the exported GPU controls (fftlib_gpu.h),
 // the twiddle/split-factor table generators and the stage-table builder.
 #include "fftlib_runtime.h"
+#include "fft_kernels.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -18,6 +19,7 @@ std::atomic<int>         g_style{FFTLIB_FUNCTION_OPTIMIZED};
 std::atomic<hipStream_t> g_stream{nullptr};
 std::atomic<int>         g_sync{1};
 std::atomic<int>         g_specialized{1};
+std::atomic<int>         g_variant{0};
 thread_local char        t_lastError[256] = "";
 
 // Device staging for host-memory callers: one grow-only buffer per role,
@@ -55,6 +57,7 @@ FFTLIB_FUNCTION_STYLE current_style() { return (FFTLIB_FUNCTION_STYLE) g_style.l
 hipStream_t           current_stream() { return g_stream.load(); }
 bool                  synchronous() { return g_sync.load() != 0; }
 bool                  specialized_kernels() { return g_specialized.load() != 0; }
+int                   c2c_variant() { return g_variant.load(); }
 
 void set_last_error(const char *what, hipError_t err)
 {
@@ -204,6 +207,15 @@ FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable)
    g_specialized.store(enable != 0);
    return FFTLIB_SUCCESS;
 }
+
+FFTLIB_STATUS FFTLIB_GPU_setC2CVariant(int32_t variant)
+{
+   if (variant < C2C_VARIANT_FP32 || variant > C2C_VARIANT_BF16X2) return FFTLIB_ERR_INVALID_VALUE;
+   g_variant.store(variant);
+   return FFTLIB_SUCCESS;
+}
+
+int32_t FFTLIB_GPU_getC2CVariant(void) { return g_variant.load(); }
 
 FFTLIB_STATUS FFTLIB_GPU_synchronize(void)
 {

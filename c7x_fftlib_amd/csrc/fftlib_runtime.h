@@ -46,6 +46,7 @@ FFTLIB_FUNCTION_STYLE current_style();
 hipStream_t           current_stream();
 bool                  synchronous();
 bool                  specialized_kernels();
+int                   c2c_variant();
 void                  set_last_error(const char *what, hipError_t err);
 
 }  // namespace fftlib

@@ -39,6 +39,25 @@ FFTLIB_STATUS FFTLIB_GPU_setSynchronous(int32_t synchronous);
  * tuning and for testing the generic kernels at every length. */
 FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable);
 
+/* Arithmetic of the fp32 complex FFT kernels (batched and single, forward and
+ * inverse) for lengths that have a DFT-as-GEMM matrix-core kernel (N = 960);
+ * other lengths always use FFTLIB_C2C_VARIANT_FP32.
+ *   FFTLIB_C2C_VARIANT_FP32   (0, default) mixed-radix fp32, the reference's
+ *                             algorithm and twiddle table;
+ *   FFTLIB_C2C_VARIANT_BF16X3 (1) radix-15 and radix-64 DFT stages as MFMA
+ *                             GEMMs on bf16 hi+mid+lo operand parts (6
+ *                             products), FP32 accumulation;
+ *   FFTLIB_C2C_VARIANT_FP16X2 (2) same with fp16 hi+lo parts (3 products);
+ *   FFTLIB_C2C_VARIANT_BF16X2 (3) same with bf16 hi+lo parts (3 products).
+ * The MFMA variants generate their own DFT matrices and do not read pW.
+ * Process wide; returns FFTLIB_ERR_INVALID_VALUE for an unknown value. */
+#define FFTLIB_C2C_VARIANT_FP32   0
+#define FFTLIB_C2C_VARIANT_BF16X3 1
+#define FFTLIB_C2C_VARIANT_FP16X2 2
+#define FFTLIB_C2C_VARIANT_BF16X2 3
+FFTLIB_STATUS FFTLIB_GPU_setC2CVariant(int32_t variant);
+int32_t       FFTLIB_GPU_getC2CVariant(void);
+
 /* Waits for all work the library enqueued on its stream. */
 FFTLIB_STATUS FFTLIB_GPU_synchronize(void);
 

@@ -209,3 +209,55 @@ def test_real_single(n):
     assert s == 0, G.F.last_error()
     refz = oracle.c2r(y, n, 1, single=True)
     assert G.rel_l2(z, refz) <= REL_TOL
+
+
+# Split-precision MFMA variants of the batched / single C2C (N = 960): error
+# against an FP64 DFT. FP16x2 and BF16x3 meet the 2e-6 bar; BF16x2 (3 bf16
+# products) is reported with its own, looser bound.
+MFMA_VARIANTS = {1: 2e-6, 2: 2e-6, 3: 1e-5}
+
+
+@pytest.fixture
+def c2c_variant():
+    lib = G.lib()
+    yield lambda v: lib.FFTLIB_GPU_setC2CVariant(v)
+    assert lib.FFTLIB_GPU_setC2CVariant(0) == 0
+
+
+@pytest.mark.parametrize("kind", ["fft", "ifft", "ifft_single"])
+@pytest.mark.parametrize("variant", sorted(MFMA_VARIANTS))
+def test_c2c_mfma_variants(variant, kind, c2c_variant):
+    assert c2c_variant(variant) == 0
+    n = 960
+    single = kind == "ifft_single"
+    b = 1 if single else 77
+    x = cplx(rng(variant * 31 + len(kind)), b, n, 1.0)
+    s, y = G.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert s == 0, G.F.last_error()
+    x64 = x.astype(np.complex128)
+    ref = np.fft.ifft(x64, axis=1) if kind != "fft" else np.fft.fft(x64, axis=1)
+    assert G.rel_l2(y, ref) <= MFMA_VARIANTS[variant]
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_c2c_mfma_scaled_inputs(variant, c2c_variant):
+    """Inputs far from unit magnitude (fp16 parts need the power-of-two scaling)."""
+    assert c2c_variant(variant) == 0
+    n, b = 960, 6
+    x = cplx(rng(99 + variant), b, n, 1.0) * np.array([1e-6, 1e-3, 1.0, 3e2, 6e4, 2e7], np.float32)[:, None]
+    s, y = G.c2c(x.astype(np.complex64), n, b)
+    assert s == 0, G.F.last_error()
+    ref = np.fft.fft(x.astype(np.complex128), axis=1)
+    for i in range(b):
+        assert G.rel_l2(y[i], ref[i]) <= 2e-6, i
+
+
+def test_c2c_variant_other_lengths_use_fp32(c2c_variant):
+    """Lengths without an MFMA kernel keep the fp32 path (bit-identical results)."""
+    n, b = 1024, 3
+    x = cplx(rng(5), b, n, 1.0)
+    s0, y0 = G.c2c(x, n, b)
+    assert c2c_variant(2) == 0
+    s1, y1 = G.c2c(x, n, b)
+    assert s0 == 0 and s1 == 0
+    assert np.array_equal(y0.view(np.uint32), y1.view(np.uint32))
