@@ -1,34 +1,39 @@
 // Split-precision MFMA variant of the batched complex FFT, N = 960.
 //
-// The 960-point DFT is computed as two dense DFT stages on the matrix cores,
-// with n = 64a + b (a < 15, b < 64) and k = c + 15d (c < 15, d < 64):
+// The 960-point DFT is computed with n = 64a + b (a < 15, b < 64),
+// b = 4 b1 + b2 (b1 < 16, b2 < 4) and k = c + 15 d1 + 240 d2
+// (c < 15, d1 < 16, d2 < 4):
 //
-//   Y1[c][b] = sum_a W15^(ac) x[64a + b]            radix-15 stage (GEMM 1)
-//   Y2[c][b] = W960^(bc) Y1[c][b]                    twiddle (fp32 VALU)
-//   X[c + 15d] = sum_b Y2[c][b] W64^(bd)             radix-64 stage (GEMM 2)
+//   Y1[c][b]     = sum_a W15^(ac) x[64a + b]             radix-15 stage (GEMM 1)
+//   Y2[c][b]     = W960^(bc) Y1[c][b]                     twiddle
+//   V[c][b2][d1] = sum_b1 W16^(b1 d1) Y2[c][4 b1 + b2]    radix-16 stage (GEMM 2)
+//   X[c + 15 d1 + 240 d2] = sum_b2 W4^(b2 d2) W64^(b2 d1) V[c][b2][d1]
+//                                                        radix-4 stage (fp32 VALU)
 //
-// Both GEMMs run in real form on v_mfma_f32_16x16x32_{f16,bf16} with FP32
-// accumulation. Operands are split into sums of 16-bit parts so the products
-// recover FP32-level accuracy:
+// The radix-15 and radix-16 DFT stages run in real form on
+// v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation; their DFT matrices
+// (32 x 32 real) stay in registers in MFMA operand layout. The data operands
+// are split into sums of 16-bit parts so that the products recover
+// FP32-level accuracy:
 //   FP16x2: x = h + l (fp16), products h*h + h*l + l*h;
 //   BF16x2: same with bf16 parts;
 //   BF16x3: x = h + m + l (bf16), products hh + hm + mh + hl + lh + mm.
 // FP16 parts are taken of the transform scaled by a power of two (exact) so
-// that its largest input is near 1: fp16 range then holds every operand.
+// that its largest input is in [0.5, 1): fp16 range then holds every operand.
+// The twiddles and the radix-4 stage are fp32.
 //
 // This is a different (but equivalent) algorithm from the reference's
 // in-place mixed-radix DIF (src/fft_c7x/FFTLIB_fft1dBatched_i32fc_c32fc_o32fc/
 // ..._cn.c:240-491): it does not read the caller's twiddle table; the DFT
-// matrices and W960 twiddles are generated here in double precision. It is
+// matrices and twiddles are generated here in double precision. It is
 // selected with FFTLIB_GPU_setC2CVariant and its error against an FP64 DFT
 // is a property of the split scheme.
 //
 // Work decomposition: 256-thread workgroups, persistent; each wavefront owns
-// one 8.25 KB LDS region and runs whole transforms in it (transform
+// one 9 KB LDS region and runs whole transforms in it (transform
 // w, w + 4*gridDim.x, ...), prefetching the next transform's input into
-// registers while the current one goes through the two GEMMs. The DFT-matrix
-// fragments (already in MFMA operand layout) and the twiddle table are
-// copied to LDS once per workgroup.
+// registers while the current one is transformed. The W960 table is copied
+// to LDS once per workgroup.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -48,11 +53,11 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 constexpr int      kN        = 960;
 constexpr int      kWaves    = 4;
 constexpr int      kThreadsM = 64 * kWaves;
-constexpr uint32_t kXRow     = 66;                 // complex slots per staged input row (64 + pad)
-constexpr uint32_t kY2Row    = 132;                // floats per Y2 row (128 + pad)
-constexpr uint32_t kRegion   = 16 * kY2Row;        // floats per wave region (>= 15*kXRow*2 and 2*960)
-constexpr int      kFragA1   = 2 * 64 * 8;         // halves of the two radix-15 A fragments, per part
-constexpr int      kFragB2   = 2 * 2 * 4 * 64 * 8; // halves of the radix-64 B fragments (Fr, Fi), per part
+constexpr uint32_t kXRow     = 66;          // complex slots per staged input row (64 + pad)
+constexpr uint32_t kVRow     = 36;          // floats per GEMM-2 operand row (re[16] im[16] + pad)
+constexpr uint32_t kRegion   = 64 * kVRow;  // floats per wave region
+constexpr int      kFragA1   = 2 * 64 * 8;  // halves of the two radix-15 A fragments, per part
+constexpr int      kFragB16  = 2 * 64 * 8;  // halves of the two radix-16 B fragments, per part
 static_assert(kRegion >= 15 * kXRow * 2 && kRegion >= 2 * kN, "wave region too small");
 
 enum Split : int { SPLIT_FP16X2 = 0, SPLIT_BF16X2 = 1, SPLIT_BF16X3 = 2 };
@@ -135,7 +140,7 @@ __device__ __forceinline__ u16x8 neg8(u16x8 v) { return v ^ (uint16_t) 0x8000u; 
 struct MfmaArgs {
    const float2   *x;
    float2         *y;
-   const uint16_t *frag;      // [part][A1 | B2] fragment tables
+   const uint16_t *frag;      // A1 fragments of every part, then B16 fragments of every part
    const float2   *tw960;     // W960^m, m < 960
    uint32_t        numChannels;
    int             mode;      // C2CMode
@@ -154,31 +159,29 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
 {
    constexpr int P = SplitTraits<S>::parts;
    extern __shared__ float smf[];
-   uint16_t *fragA1 = (uint16_t *) smf;                          // P * kFragA1 halves
-   uint16_t *fragB2 = fragA1 + P * kFragA1;                       // P * kFragB2 halves
-   float2   *tw     = (float2 *) (fragB2 + P * kFragB2);          // 960 complex
-   float    *region = (float *) (tw + kN) + (threadIdx.x / 64) * kRegion;
-   float2   *rc     = (float2 *) region;
+   float2 *tw     = (float2 *) smf;                                       // W960^m, m < 960
+   float  *region = smf + 2 * kN + (threadIdx.x / 64) * kRegion;
+   float2 *rc     = (float2 *) region;
 
    const uint32_t lane = threadIdx.x % 64;
    const uint32_t wave = threadIdx.x / 64;
    const uint32_t q    = lane / 16, r16 = lane % 16;
 
-   // constants to LDS (once per workgroup)
-   {
-      const uint32_t *src   = (const uint32_t *) a.frag;
-      uint32_t       *dst   = (uint32_t *) fragA1;
-      const uint32_t  words = (uint32_t) P * (kFragA1 + kFragB2) / 2u;
-      for (uint32_t i = threadIdx.x; i < words; i += kThreadsM) dst[i] = src[i];
-      for (uint32_t i = threadIdx.x; i < (uint32_t) kN; i += kThreadsM) tw[i] = a.tw960[i];
-      __syncthreads();
-   }
-   // radix-15 A fragments stay in registers
-   u16x8 A1[2][P];
+   for (uint32_t i = threadIdx.x; i < (uint32_t) kN; i += kThreadsM) tw[i] = a.tw960[i];
+   // DFT-matrix fragments in registers
+   u16x8 A1[2][P], B16[2][P];
 #pragma unroll
    for (int p = 0; p < P; ++p)
 #pragma unroll
-      for (int mt = 0; mt < 2; ++mt) A1[mt][p] = *(const u16x8 *) (fragA1 + p * kFragA1 + (mt * 64 + lane) * 8);
+      for (int t = 0; t < 2; ++t) {
+         A1[t][p]  = *(const u16x8 *) (a.frag + p * kFragA1 + (t * 64 + lane) * 8);
+         B16[t][p] = *(const u16x8 *) (a.frag + P * kFragA1 + p * kFragB16 + (t * 64 + lane) * 8);
+      }
+   __syncthreads();
+   // W64^(b2 d1) for d1 = lane % 16, b2 = 1..3 (= W960^(15 b2 d1))
+   float2 w64[3];
+#pragma unroll
+   for (int b2 = 1; b2 < 4; ++b2) w64[b2 - 1] = tw[(15u * (uint32_t) b2 * r16) % (uint32_t) kN];
 
    const float sxIn  = a.mode == C2C_INVERSE_PRESCALE ? 1.0f / (float) kN : 1.0f;
    const float syIn  = a.mode == C2C_FORWARD ? sxIn : -sxIn;
@@ -193,7 +196,7 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
       for (int j = 0; j < 15; ++j) xin[j] = a.x[(size_t) f * kN + lane + 64 * j];
    }
    for (; f < a.numChannels; f += stride) {
-      // ---- stage the input (x[64a + b] at row a, column b), scaled ----------
+      // ---- stage the input (x[64a + b] at row a, column b) --------------------
       float amax = 0.0f;
 #pragma unroll
       for (int j = 0; j < 15; ++j) {
@@ -202,7 +205,7 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
          rc[j * kXRow + lane] = v;
       }
       // Power-of-two scale bringing max|x| into [0.5, 1) for the fp16 parts
-      // (1 when the transform is zero or for the bf16 schemes).
+      // (1 when the transform is zero or not finite, and for bf16).
       float scale = 1.0f, unscale = 1.0f;
       if constexpr (S == SPLIT_FP16X2) {
          amax = wave_max(amax);
@@ -213,13 +216,13 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
             unscale = ldexpf(1.0f, e);
          }
       }
-      // ---- prefetch the next transform --------------------------------------
+      // ---- prefetch the next transform ------------------------------------------
       const uint32_t fn = f + stride;
       if (fn < a.numChannels) {
 #pragma unroll
          for (int j = 0; j < 15; ++j) xin[j] = a.x[(size_t) fn * kN + lane + 64 * j];
       }
-      // ---- GEMM 1: C1 (32 x 64) = A1 (32 x 32) * B1 (32 x 64) ----------------
+      // ---- GEMM 1: C1 (32 x 64) = A1 (32 x 32) * B1 (32 x 64) -------------------
       // B1 row k: k < 15 -> Re x[a = k][b], 16 <= k < 31 -> Im x[a = k - 16][b].
       f32x4 acc1[2][4];
 #pragma unroll
@@ -237,61 +240,60 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
 #pragma unroll
          for (int mt = 0; mt < 2; ++mt) acc1[mt][nt] = split_mma<S>(A1[mt], B, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
       }
-      // ---- twiddle, Y2 to LDS (row c: 64 real parts, then 64 imaginary) --------
+      // ---- twiddle W960^(bc); Y2 to LDS as rows (c, b2): re[b1], im[b1] ---------
 #pragma unroll
       for (int nt = 0; nt < 4; ++nt) {
 #pragma unroll
          for (int i = 0; i < 4; ++i) {
-            const uint32_t c  = 4u * q + (uint32_t) i;   // row 15 is padding (zero)
-            const uint32_t b  = 16u * nt + r16;
-            const float2   w  = tw[(b * c) % (uint32_t) kN];
-            const float    yr = acc1[0][nt][i], yi = acc1[1][nt][i];
-            region[c * kY2Row + b]      = yr * w.x - yi * w.y;
-            region[c * kY2Row + 64 + b] = yr * w.y + yi * w.x;
+            const uint32_t c   = 4u * q + (uint32_t) i;   // row 15 is padding (zero)
+            const uint32_t b   = 16u * nt + r16;
+            const float2   w   = tw[(b * c) % (uint32_t) kN];
+            const float    yr  = acc1[0][nt][i], yi = acc1[1][nt][i];
+            float         *row = region + (4u * c + (b & 3u)) * kVRow + (b >> 2);
+            row[0]             = yr * w.x - yi * w.y;
+            row[16]            = yr * w.y + yi * w.x;
          }
       }
-      // ---- GEMM 2: Z (16 x 128) = A2 (16 x 128) * B2 (128 x 128) -------------
-      u16x8 A2[4][P];
+      // ---- GEMM 2 (radix 16), twiddle W64, radix 4, Z to LDS in output order ---
+      // Row m = 16 mt + lane % 16 of the operand is (c = 4 mt + lane % 16 / 4,
+      // b2 = lane % 4); output rows 16 mt + 4 q + i are (c = 4 mt + q, b2 = i).
+      f32x4 vr[4], vi[4];
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-         float          v[8];
-         const float4  *src = (const float4 *) (region + r16 * kY2Row + 32 * ks + 8 * q);
-         const float4   lo = src[0], hi = src[1];
-         v[0] = lo.x, v[1] = lo.y, v[2] = lo.z, v[3] = lo.w, v[4] = hi.x, v[5] = hi.y, v[6] = hi.z, v[7] = hi.w;
-         split8<S>(v, A2[ks]);
+      for (int mt = 0; mt < 4; ++mt) {
+         const float4 *src = (const float4 *) (region + (16u * mt + r16) * kVRow + 8u * q);
+         const float4  lo = src[0], hi = src[1];
+         float         v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};   // already scaled
+         u16x8         A[P];
+         split8<S>(v, A);
+         vr[mt] = split_mma<S>(A, B16[0], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+         vi[mt] = split_mma<S>(A, B16[1], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
       }
-      f32x4 acc2[8];
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int mt = 0; mt < 4; ++mt) {
+         const uint32_t c = 4u * mt + q;
+         if (c >= 15u) continue;
+         // t[b2] = W64^(b2 d1) V[c][b2][d1], d1 = lane % 16
+         float2 t[4];
+         t[0] = make_float2(vr[mt][0], vi[mt][0]);
 #pragma unroll
-         for (int ks = 0; ks < 4; ++ks) {
-            // B2 block: rows from Re Y2 (ks < 2) or Im Y2; columns to Re Z
-            // (nt < 4) or Im Z: [[Fr, Fi], [-Fi, Fr]].
-            const bool useFi = (ks < 2) != (nt < 4);
-            const bool neg   = ks >= 2 && nt < 4;
-            const int  mat   = useFi ? 1 : 0;
-            u16x8      B[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-               const u16x8 w = *(const u16x8 *) (fragB2 + p * kFragB2 +
-                                                 (((mat * 2 + (ks & 1)) * 4 + (nt & 3)) * 64 + lane) * 8);
-               B[p]          = neg ? neg8(w) : w;
-            }
-            c = split_mma<S>(A2[ks], B, c);
+         for (int b2 = 1; b2 < 4; ++b2) {
+            const float2 w = w64[b2 - 1];
+            t[b2]          = make_float2(vr[mt][b2] * w.x - vi[mt][b2] * w.y, vr[mt][b2] * w.y + vi[mt][b2] * w.x);
          }
-         acc2[nt] = c;
-      }
-      // ---- Z to LDS in output order k = c + 15d, then coalesced stores --------
+         const float2 s02 = make_float2(t[0].x + t[2].x, t[0].y + t[2].y);
+         const float2 d02 = make_float2(t[0].x - t[2].x, t[0].y - t[2].y);
+         const float2 s13 = make_float2(t[1].x + t[3].x, t[1].y + t[3].y);
+         const float2 d13 = make_float2(t[1].x - t[3].x, t[1].y - t[3].y);
+         float2       z[4];
+         z[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+         z[1] = make_float2(d02.x + d13.y, d02.y - d13.x);   // d02 - i d13
+         z[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+         z[3] = make_float2(d02.x - d13.y, d02.y + d13.x);   // d02 + i d13
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-#pragma unroll
-         for (int i = 0; i < 4; ++i) {
-            const uint32_t c = 4u * q + (uint32_t) i;
-            const uint32_t d = 16u * nt + r16;
-            if (c >= 15u) continue;
-            const float zr = unscale * acc2[nt][i], zi = unscale * acc2[nt + 4][i];
-            rc[c + 15u * d] = pdiv ? make_float2(zr / (float) kN, -(zi / (float) kN)) : make_float2(zr, syOut * zi);
+         for (int d2 = 0; d2 < 4; ++d2) {
+            const float zr = unscale * z[d2].x, zi = unscale * z[d2].y;
+            rc[c + 15u * r16 + 240u * (uint32_t) d2] =
+               pdiv ? make_float2(zr / (float) kN, -(zi / (float) kN)) : make_float2(zr, syOut * zi);
          }
       }
       float2 *yo = a.y + (size_t) f * kN;
@@ -339,44 +341,42 @@ struct DeviceTables {
 std::mutex   g_tabMutex;
 DeviceTables g_tables[16];
 
-// A1 (32 x 32, real form of the 15-point DFT) and B2 (Fr, Fi of the 64-point
-// DFT) in MFMA operand layout for lane l: A[m = l % 16][k = 8 (l / 16) + i],
-// B[k = 8 (l / 16) + i][n = l % 16].
+// A1 (32 x 32, real form of the 15-point DFT; rows/columns 15 and 31 are
+// padding) and B16 (32 x 32, real form of the 16-point DFT) in MFMA operand
+// layout for lane l: A[m = 16 t + l % 16][k = 8 (l / 16) + i],
+// B[k = 8 (l / 16) + i][n = 16 t + l % 16]. Layout: A1 of every part, then
+// B16 of every part.
 hipError_t build_tables(int S, uint16_t **out)
 {
-   const int P      = S == SPLIT_BF16X3 ? 3 : 2;
-   const size_t cnt = (size_t) P * (kFragA1 + kFragB2);
+   const int    P   = S == SPLIT_BF16X3 ? 3 : 2;
+   const size_t cnt = (size_t) P * (kFragA1 + kFragB16);
    uint16_t    *h   = new uint16_t[cnt];
    const double pi  = 3.14159265358979323846;
    uint16_t     parts[3];
-   auto A1 = [&](int m, int k) -> double {   // real form of F15 (rows/cols 15, 31 padding)
+   auto A1 = [&](int m, int k) -> double {
       const int c = m & 15, a = k & 15;
       if (c >= 15 || a >= 15) return 0.0;
       const double ang = -2.0 * pi * (double) ((a * c) % 15) / 15.0;
       const double fr = cos(ang), fi = sin(ang);
-      if (m < 16) return k < 16 ? fr : -fi;
+      if (m < 16) return k < 16 ? fr : -fi;   // Re out = fr Re in - fi Im in
+      return k < 16 ? fi : fr;                // Im out = fi Re in + fr Im in
+   };
+   auto B16 = [&](int k, int n) -> double {
+      const int    b1 = k & 15, d1 = n & 15;
+      const double ang = -2.0 * pi * (double) ((b1 * d1) % 16) / 16.0;
+      const double fr = cos(ang), fi = sin(ang);
+      if (n < 16) return k < 16 ? fr : -fi;
       return k < 16 ? fi : fr;
    };
-   for (int mt = 0; mt < 2; ++mt)
+   for (int t = 0; t < 2; ++t)
       for (int l = 0; l < 64; ++l)
          for (int i = 0; i < 8; ++i) {
-            host_split(S, A1(16 * mt + l % 16, 8 * (l / 16) + i), parts);
-            for (int p = 0; p < P; ++p) h[(size_t) p * kFragA1 + (mt * 64 + l) * 8 + i] = parts[p];
+            host_split(S, A1(16 * t + l % 16, 8 * (l / 16) + i), parts);
+            for (int p = 0; p < P; ++p) h[(size_t) p * kFragA1 + (t * 64 + l) * 8 + i] = parts[p];
+            host_split(S, B16(8 * (l / 16) + i, 16 * t + l % 16), parts);
+            for (int p = 0; p < P; ++p)
+               h[(size_t) P * kFragA1 + (size_t) p * kFragB16 + (t * 64 + l) * 8 + i] = parts[p];
          }
-   uint16_t *hb = h + (size_t) P * kFragA1;
-   for (int mat = 0; mat < 2; ++mat)
-      for (int kh = 0; kh < 2; ++kh)
-         for (int nt = 0; nt < 4; ++nt)
-            for (int l = 0; l < 64; ++l)
-               for (int i = 0; i < 8; ++i) {
-                  const int    b   = 32 * kh + 8 * (l / 16) + i;
-                  const int    d   = 16 * nt + l % 16;
-                  const double ang = -2.0 * pi * (double) ((b * d) % 64) / 64.0;
-                  host_split(S, mat == 0 ? cos(ang) : sin(ang), parts);
-                  for (int p = 0; p < P; ++p)
-                     hb[(size_t) p * kFragB2 + (((mat * 2 + kh) * 4 + nt) * 64 + l) * 8 + i] = parts[p];
-               }
-   // layout: all A1 parts, then all B2 parts (copied to LDS as one block)
    uint16_t  *d = nullptr;
    hipError_t e = hipMalloc((void **) &d, cnt * sizeof(uint16_t));
    if (e == hipSuccess) e = hipMemcpy(d, h, cnt * sizeof(uint16_t), hipMemcpyHostToDevice);
@@ -427,8 +427,7 @@ template <int S>
 hipError_t launch_s(const MfmaArgs &a, hipStream_t stream)
 {
    constexpr int P   = SplitTraits<S>::parts;
-   const size_t  lds = (size_t) P * (kFragA1 + kFragB2) * sizeof(uint16_t) + kN * sizeof(float2) +
-                      (size_t) kWaves * kRegion * sizeof(float);
+   const size_t  lds = kN * sizeof(float2) + (size_t) kWaves * kRegion * sizeof(float);
    static int resident = 0;
    if (lds > 65536) {
       const hipError_t e = hipFuncSetAttribute((const void *) c2c960_mfma_kernel<S>,
