@@ -261,8 +261,8 @@ __device__ __forceinline__ float2 fixed_input(const float2 *in, uint32_t i, cons
          v1   = in[N - i];
          v1.y = -v1.y;
       }
-      const float2 A = *(const float2 *) (a.sf + 2u * i);
-      const float2 B = *(const float2 *) (a.sf + 2u * N + 2u * i);
+      const float2 A = make_float2(a.sf[2u * i], a.sf[2u * i + 1u]);
+      const float2 B = make_float2(a.sf[2u * N + 2u * i], a.sf[2u * N + 2u * i + 1u]);
       return make_float2(v0.x * A.x - v0.y * A.y + (v1.x * B.x - v1.y * B.y),
                          v0.x * A.y + v0.y * A.x + (v1.x * B.y + v1.y * B.x));
    }
@@ -416,8 +416,8 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
             const uint32_t i = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || i >= (uint32_t) N)) break;
             const float2 zi = v[c], zm = vm[c];
-            const float2 A  = *(const float2 *) (a.sf + 2u * i);
-            const float2 B  = *(const float2 *) (a.sf + 4u * N - 2u * i - 2u);
+            const float2 A  = make_float2(a.sf[2u * i], a.sf[2u * i + 1u]);
+            const float2 B  = make_float2(a.sf[4u * N - 2u * i - 2u], a.sf[4u * N - 2u * i - 1u]);
             float2       t1;
             if (i == 0u) t1 = make_float2(zi.x * B.x + zi.y * B.y, zi.x * B.x - zi.y * B.y);
             else t1 = make_float2(zm.x * B.x + zm.y * B.y, zm.x * B.y - zm.y * B.x);

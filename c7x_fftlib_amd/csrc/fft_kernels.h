@@ -57,6 +57,16 @@ hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uin
 // Largest complex length the single-pass LDS kernels accept.
 uint32_t max_lds_points();
 
+// True when one transform of st.n points (with padding) fits one workgroup's
+// LDS, i.e. launch_c2c / launch_r2c / launch_c2r can run it.
+bool fits_lds(const StageTable &st);
+
+// Complex FFTs longer than one workgroup's LDS (fft_large.hip): outer radix
+// passes over HBM into `scratch` (st.n * numChannels complex values), then
+// sub-blocks finished in LDS. Same modes as launch_c2c.
+hipError_t launch_c2c_large(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
+                            int mode, float *scratch, hipStream_t stream);
+
 // Lengths with a compile-time specialised complex kernel (fft_fixed.hip).
 bool fixed_c2c_supported(uint32_t n);
 

@@ -376,6 +376,12 @@ hipError_t launch(K kernel, const Geometry &g, uint32_t numChannels, hipStream_t
 
 uint32_t max_lds_points() { return max_lds_bytes() / 16u; }
 
+bool fits_lds(const StageTable &st)
+{
+   Geometry g;
+   return pick_geometry(st.n, 8u, st.twCount, 1u, &g);
+}
+
 hipError_t launch_c2c(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
                       int mode, hipStream_t stream)
 {

@@ -57,6 +57,15 @@ FFTLIB_STATUS write_plan(void *pBlock, KernelId kernel, uint32_t numPoints, uint
    return FFTLIB_SUCCESS;
 }
 
+// The fused real-transform and int16 kernels keep a whole transform in one
+// workgroup's LDS; longer ones are refused at _kernel time.
+FFTLIB_STATUS refuse_too_long(const Plan &p)
+{
+   if (fits_lds(p.st)) return FFTLIB_SUCCESS;
+   set_last_error_text("transform longer than one workgroup's LDS: only the complex (C2C/IFFT) kernels support it");
+   return FFTLIB_ERR_NOT_IMPLEMENTED;
+}
+
 // Reads and validates the plan an _init call left in pBlock.
 FFTLIB_STATUS read_plan(const void *pBlock, KernelId kernel, uint32_t numPoints, Plan *p)
 {
@@ -80,6 +89,13 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    if (specialized_kernels() && fixed_c2c_supported(p.st.n))
       return launch_c2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st.n, p.numChannels,
                               mode, s);
+   if (!fits_lds(p.st)) {
+      void      *scratch = nullptr;
+      hipError_t e       = device_scratch((size_t) p.st.n * p.numChannels * 8u, &scratch);
+      if (e != hipSuccess) return e;
+      return launch_c2c_large((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st, p.numChannels, mode,
+                              (float *) scratch, s);
+   }
    return launch_c2c((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st, p.numChannels, mode, s);
 }
 
@@ -130,6 +146,7 @@ FFTLIB_STATUS exec_i16(KernelId kernel, const int16_t *pX, const int16_t *pW, in
    FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
    if (st != FFTLIB_SUCCESS) return st;
    if (pX == nullptr || pW == nullptr || pY == nullptr || pShift == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
    if (p.style == FFTLIB_FUNCTION_GPU &&
        (misaligned(pX, 4) || misaligned(pW, 4) || misaligned(pY, 4) || misaligned(pShift, 4)))
       return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
@@ -153,6 +170,7 @@ FFTLIB_STATUS exec_r2c(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *
    FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
    if (st != FFTLIB_SUCCESS) return st;
    if (outmode > 1u) return FFTLIB_ERR_INVALID_OUTMODE;
+   if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
    if (pX == nullptr || pW == nullptr || pSf == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
    if (p.style == FFTLIB_FUNCTION_GPU &&
        (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pSf, 4)))
@@ -178,6 +196,7 @@ FFTLIB_STATUS exec_c2r(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *
    FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
    if (st != FFTLIB_SUCCESS) return st;
    if (inMode > 1u) return FFTLIB_ERR_INVALID_VALUE;
+   if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
    if (pX == nullptr || pW == nullptr || pSf == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
    if (p.style == FFTLIB_FUNCTION_GPU &&
        (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pSf, 4)))

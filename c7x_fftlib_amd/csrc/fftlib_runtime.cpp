@@ -64,6 +64,37 @@ void set_last_error(const char *what, hipError_t err)
    snprintf(t_lastError, sizeof t_lastError, "%s: %s", what, hipGetErrorString(err));
 }
 
+void set_last_error_text(const char *text) { snprintf(t_lastError, sizeof t_lastError, "%s", text); }
+
+hipError_t device_scratch(size_t bytes, void **ptr)
+{
+   struct Slot {
+      void  *p   = nullptr;
+      size_t cap = 0;
+   };
+   static std::mutex mu;
+   static Slot       slots[16];
+   int               dev = 0;
+   hipError_t        e   = hipGetDevice(&dev);
+   if (e != hipSuccess) return e;
+   if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+   std::lock_guard<std::mutex> lock(mu);
+   Slot &s = slots[dev];
+   if (bytes > s.cap) {
+      if (s.p) {
+         // a kernel of an earlier call may still read the old buffer
+         if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+         if ((e = hipFree(s.p)) != hipSuccess) return e;
+         s.p = nullptr, s.cap = 0;
+      }
+      const size_t want = (bytes + 65535) & ~(size_t) 65535;
+      if ((e = hipMalloc(&s.p, want)) != hipSuccess) return e;
+      s.cap = want;
+   }
+   *ptr = s.p;
+   return hipSuccess;
+}
+
 static FFTLIB_STATUS fail(const char *what, hipError_t err)
 {
    set_last_error(what, err);

@@ -48,5 +48,11 @@ bool                  synchronous();
 bool                  specialized_kernels();
 int                   c2c_variant();
 void                  set_last_error(const char *what, hipError_t err);
+void                  set_last_error_text(const char *text);
+
+// Library-owned device scratch of at least `bytes` on the current device
+// (grow-only, kept until the process ends). Calls that use it on one device
+// must not overlap in time; the GPU controls document this.
+hipError_t device_scratch(size_t bytes, void **ptr);
 
 }  // namespace fftlib

@@ -261,3 +261,27 @@ def test_c2c_variant_other_lengths_use_fp32(c2c_variant):
     s1, y1 = G.c2c(x, n, b)
     assert s0 == 0 and s1 == 0
     assert np.array_equal(y0.view(np.uint32), y1.view(np.uint32))
+
+
+# Complex lengths longer than one workgroup's LDS: outer radix passes over HBM
+# plus sub-blocks finished in LDS.
+LARGE_SIZES = [20000, 30720, 32768, 40000, 50000, 59049, 65536]
+
+
+@pytest.mark.parametrize("kind", ["fft", "ifft", "ifft_single"])
+@pytest.mark.parametrize("n", LARGE_SIZES)
+def test_c2c_large(n, kind):
+    single = kind == "ifft_single"
+    b = 1 if single else 3
+    x = cplx(rng(n + len(kind)), b, n, 1.0)
+    s, y = G.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert s == 0, G.F.last_error()
+    ref = oracle.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert G.rel_l2(y, ref) <= REL_TOL
+
+
+def test_real_too_long_is_refused():
+    n = 65536
+    x = rng(3).uniform(-1, 1, (1, n)).astype(np.float32)
+    s, _ = G.r2c(x, n, 1, outmode=0)
+    assert s == G.F.FFTLIB_ERR_NOT_IMPLEMENTED
