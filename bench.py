@@ -18,7 +18,11 @@ duration. One JSON line is printed:
               on this host on a bounded sample of the same workload.
 
 Other workloads: --workload i16_256 (int16, N=256, 8192 channels),
-r2c_4096 (R2C then C2R, N=4096, 16384 channels).
+r2c_4096 (R2C then C2R, N=4096, 16384 channels), ragged_10000 (10000 C2C
+transforms of lengths drawn log-uniformly from [64, 65536] and snapped to
+2/3/5-smooth lengths, forward then inverse in one graph launch),
+copy_503m (plain device copy of the N=960 workload's bytes, for the HBM
+counter calibration).
 """
 import argparse
 import ctypes
@@ -42,7 +46,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "copy_503m"])
+    ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "ragged_10000", "copy_503m"])
+    ap.add_argument("--no-tune", action="store_true", help="ragged_10000: skip the autotuning pass")
     ap.add_argument("--variant", default="fp32", choices=["fp32", "bf16x3", "fp16x2", "bf16x2"],
                     help="C2C arithmetic: fp32 mixed radix, or MFMA DFT-as-GEMM with split bf16/fp16 operands")
     ap.add_argument("--generic", action="store_true",
@@ -162,6 +167,107 @@ class Workload:
         return res
 
     # -- plain device copy of the C2C workload's input size (counter calibration)
+    # -- ragged batch: 10000 C2C transforms, forward then inverse -----------
+    @staticmethod
+    def ragged_lengths(count, seed):
+        """Lengths log-uniform on [64, 65536], each snapped to the 2/3/5-smooth
+        length nearest in log distance (ties to the smaller)."""
+        smooth = []
+        for a in range(17):
+            for b in range(11):
+                for c in range(7):
+                    v = (2 ** a) * (3 ** b) * (5 ** c)
+                    if 16 <= v <= 65536:
+                        smooth.append(v)
+        smooth = np.array(sorted(smooth), np.float64)
+        logs = np.log(smooth)
+        r = philox(seed)
+        want = r.uniform(np.log(64.0), np.log(65536.0), count)
+        idx = np.abs(logs[None, :] - want[:, None]).argmin(axis=1)
+        return smooth[idx].astype(np.int64)
+
+    def _setup_ragged_10000(self):
+        F, lib, torch = self.F, self.lib, self.torch
+        lengths = self.ragged_lengths(10000, 10000)
+        self.lengths = lengths
+        total = int(lengths.sum())
+        self.n, self.b = 0, len(lengths)
+        arr = (ctypes.c_uint32 * len(lengths))(*[int(v) for v in lengths])
+        self.h = ctypes.c_void_p()
+        st = lib.FFTLIB_GPU_raggedCreate(ctypes.byref(self.h), arr, len(lengths))
+        if st != 0:
+            raise RuntimeError(f"raggedCreate status {st}: {F.last_error()}")
+        r = philox(10001)
+        host = np.empty(total, np.complex64)
+        chunk = 1 << 24
+        for i in range(0, total, chunk):
+            m = min(chunk, total - i)
+            host[i:i + m].real = r.uniform(-1, 1, m)
+            host[i:i + m].imag = r.uniform(-1, 1, m)
+        self.host_x = host
+        self.x = torch.from_numpy(host.view(np.float32)).to("cuda:0")
+        self.y = torch.empty_like(self.x)
+        self.z = torch.empty_like(self.x)
+        if not self.args.no_tune:
+            st = lib.FFTLIB_GPU_raggedTune(self.h, self.x.data_ptr(), self.y.data_ptr())
+            if st != 0:
+                raise RuntimeError(f"raggedTune status {st}: {F.last_error()}")
+        lg = np.log2(lengths.astype(np.float64))
+        self.flops = float(2 * (5.0 * lengths * lg).sum())
+        self.bytes = float(2 * 16.0 * total)
+        self.dtype = "f32"
+        self.unit = "GFLOP/s"
+        self.config = {"workload": "ragged batch of 10000 C2C FFTs, lengths log-uniform on [64, 65536] snapped to "
+                                   "2/3/5-smooth (seed 10000), forward then inverse in one graph launch",
+                       "transforms": int(len(lengths)), "distinct_lengths": int(len(np.unique(lengths))),
+                       "points": total, "tuned": not self.args.no_tune,
+                       "input": "complex64 uniform[-1,1) Philox seed 10001"}
+        self.kernel_name = "ragged graph (specialised C2C kernels <= 16384 points, outer-pass path above)"
+        xp, yp, zp, h = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(), self.h
+        fn = lib.FFTLIB_GPU_raggedExec
+
+        def step():
+            s = fn(h, xp, yp, zp, 2)
+            if s != 0:
+                raise RuntimeError(f"raggedExec status {s}: {F.last_error()}")
+        self.step = step
+
+    def check_ragged_10000(self):
+        """Forward output of 64 transforms against the FP64 FFT; round-trip error
+        of the whole batch."""
+        idx = np.linspace(0, len(self.lengths) - 1, 64).astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(self.lengths)[:-1]])
+        y = self.y.cpu().numpy().view(np.complex64)
+        num = den = 0.0
+        for i in idx:
+            o, n = int(offs[i]), int(self.lengths[i])
+            ref = np.fft.fft(self.host_x[o:o + n].astype(np.complex128))
+            num += float(np.sum(np.abs(y[o:o + n] - ref) ** 2))
+            den += float(np.sum(np.abs(ref) ** 2))
+        z = self.z.cpu().numpy().view(np.complex64)
+        rt = float(np.linalg.norm(z - self.host_x) / np.linalg.norm(self.host_x))
+        return {"fwd_rel_l2_vs_fp64": math.sqrt(num / den), "roundtrip_rel_l2": rt, "checked_transforms": 64}
+
+    def cpu_ragged_10000(self, seconds):
+        import oracle
+        offs = np.concatenate([[0], np.cumsum(self.lengths)[:-1]])
+        order = philox(7).permutation(len(self.lengths))
+        done_flops, t_total, count = 0.0, 0.0, 0
+        for i in order:
+            o, n = int(offs[i]), int(self.lengths[i])
+            x = self.host_x[o:o + n][None, :]
+            t0 = time.perf_counter()
+            y = oracle.c2c(x, n, 1)
+            oracle.c2c(y, n, 1, inverse=True)
+            t_total += time.perf_counter() - t0
+            done_flops += 2 * 5.0 * n * math.log2(n)
+            count += 1
+            if t_total > seconds:
+                break
+        return {"value": done_flops / t_total / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+                "sample": f"oracle C2C forward + inverse (natural-C restatement), {count} of the 10000 transforms "
+                          f"in random order, {t_total:.2f} s"}
+
     def _setup_copy_503m(self):
         torch = self.torch
         self.n, self.b = 960, 65536

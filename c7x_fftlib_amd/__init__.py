@@ -96,6 +96,13 @@ _EXTRA = {
     "FFTLIB_GPU_setSpecializedKernels": (_S, [ctypes.c_int32]),
     "FFTLIB_GPU_setC2CVariant": (_S, [ctypes.c_int32]),
     "FFTLIB_GPU_getC2CVariant": (ctypes.c_int32, []),
+    "FFTLIB_GPU_raggedCreate": (_S, [_P, _P, ctypes.c_uint32]),
+    "FFTLIB_GPU_raggedExec": (_S, [_P, _P, _P, _P, ctypes.c_int32]),
+    "FFTLIB_GPU_raggedTune": (_S, [_P, _P, _P]),
+    "FFTLIB_GPU_raggedDestroy": (_S, [_P]),
+    "FFTLIB_GPU_raggedPoints": (ctypes.c_uint64, [_P]),
+    "FFTLIB_GPU_tuneSave": (_S, [ctypes.c_char_p]),
+    "FFTLIB_GPU_tuneLoad": (_S, [ctypes.c_char_p]),
     "FFTLIB_GPU_synchronize": (_S, []),
     "FFTLIB_GPU_getLastError": (ctypes.c_char_p, []),
     "FFTLIB_GPU_version": (ctypes.c_char_p, []),
@@ -112,7 +119,7 @@ _lib = None
 
 def declared_symbols():
     """Names of every function declared in the repository's include/*.h."""
-    pat = re.compile(r"^\s*(?:FFTLIB_STATUS|void|const char \*|FFTLIB_FUNCTION_STYLE|int32_t|uint32_t)\s*\**\s*"
+    pat = re.compile(r"^\s*(?:FFTLIB_STATUS|void|const char \*|FFTLIB_FUNCTION_STYLE|int32_t|uint32_t|uint64_t)\s*\**\s*"
                      r"(\w+)\s*\(", re.M)
     names = set()
     for fname in sorted(os.listdir(INCLUDE_DIR)):

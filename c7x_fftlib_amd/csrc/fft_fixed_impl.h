@@ -1,0 +1,621 @@
+// Length-specialised gfx950 kernels (templates): see fft_fixed.hip for the
+// design. Included by fft_fixed.hip and by the translation units that
+// instantiate further lengths (fft_fixed_ext*.hip), so the instantiations
+// compile in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_engine.h"
+#include "fft_kernels.h"
+
+namespace fftlib {
+namespace fixedk {
+
+using namespace engine;
+
+inline constexpr int kFixedThreads = 256;
+inline constexpr int kMaxFixedSt   = 24;
+
+enum FixedKind : int { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
+
+// Compile-time stage schedule of a 2/3/5-smooth length (same stage order and
+// twiddle layout as build_stage_table).
+struct FixedPlan {
+   int  n;
+   int  nStages;
+   int  radix[kMaxFixedSt];
+   int  M[kMaxFixedSt];
+   int  twOff[kMaxFixedSt];
+   int  nPasses;
+   int  passFirst[kMaxFixedSt];
+   int  passLen[kMaxFixedSt];
+   int  nOdd;              // radix-5 and radix-3 digits (lsd first), then log2 digits of 2
+   int  oddDigit[kMaxFixedSt];
+   int  rOdd;              // product of the odd digits
+   int  log2Pow;           // n = rOdd * 2^log2Pow
+   int  twCount;           // complex twiddles the stages read
+   int  T;                 // work items per transform
+   int  FPB;               // transforms per workgroup
+};
+
+constexpr inline FixedPlan make_fixed_plan(int n)
+{
+   FixedPlan p{};
+   p.n      = n;
+   int t    = n, M = n, off = 0;
+   p.rOdd   = 1;
+   while (t % 5 == 0) {
+      p.radix[p.nStages] = 5, p.M[p.nStages] = M, p.twOff[p.nStages] = off, ++p.nStages;
+      off += 4 * (M / 5), M /= 5, t /= 5;
+      p.oddDigit[p.nOdd++] = 5, p.rOdd *= 5;
+   }
+   while (t % 3 == 0) {
+      p.radix[p.nStages] = 3, p.M[p.nStages] = M, p.twOff[p.nStages] = off, ++p.nStages;
+      off += 2 * (M / 3), M /= 3, t /= 3;
+      p.oddDigit[p.nOdd++] = 3, p.rOdd *= 3;
+   }
+   while (t % 4 == 0) {
+      p.radix[p.nStages] = 4, p.M[p.nStages] = M, p.twOff[p.nStages] = off, ++p.nStages;
+      off += 3 * (M / 4), M /= 4, t /= 4;
+      p.log2Pow += 2;
+   }
+   if (t == 2) {
+      p.radix[p.nStages] = 2, p.M[p.nStages] = M, p.twOff[p.nStages] = off, ++p.nStages;
+      M /= 2, t = 1;
+      p.log2Pow += 1;
+   }
+   p.twCount = off;
+   for (int s = 0; s < p.nStages;) {
+      const bool pair        = s + 1 < p.nStages && p.radix[s] * p.radix[s + 1] <= 25;
+      p.passFirst[p.nPasses] = s;
+      p.passLen[p.nPasses]   = pair ? 2 : 1;
+      ++p.nPasses;
+      s += pair ? 2 : 1;
+   }
+   int T = 1;
+   while (T < 256 && T * 16 < n) T *= 2;
+   p.T   = T;
+   p.FPB = kFixedThreads / T;
+   return p;
+}
+
+template <int N>
+struct Fixed {
+   static constexpr FixedPlan P = make_fixed_plan(N);
+};
+
+// LDS slot of point i of a transform, and slots per transform.
+__device__ __forceinline__ uint32_t fslot(uint32_t i) { return i + (i >> 5); }
+template <int N>
+constexpr uint32_t fixed_stride()
+{
+   return (uint32_t) (N + (N - 1) / 32 + 1);
+}
+
+// Position of frequency k in the in-place DIF result: the digit reversal over
+// [5.., 3.., 2..] (lsd first) splits into the reversal of k mod rOdd over the
+// odd digits, placed above the bit reversal of k / rOdd.
+template <int N>
+__device__ __forceinline__ uint32_t fixed_position(uint32_t k)
+{
+   constexpr FixedPlan P = Fixed<N>::P;
+   constexpr int       A = P.log2Pow;
+   uint32_t            m = k, ro = 0;
+   if constexpr (P.rOdd > 1) {
+      m          = k / (uint32_t) P.rOdd;
+      uint32_t r = k - m * (uint32_t) P.rOdd;
+#pragma unroll
+      for (int i = 0; i < P.nOdd; ++i) {
+         const uint32_t d  = (uint32_t) P.oddDigit[i];
+         const uint32_t qd = r / d;
+         ro                = ro * d + (r - qd * d);
+         r                 = qd;
+      }
+   }
+   if constexpr (A == 0) return ro;
+   else return (ro << A) | (__brev(m) >> (32 - A));
+}
+
+// One work item of register pass PI: the R1*R2 points d*M + p2 + q2*j of
+// item u (d = u / q2, p2 = u % q2), both stages in registers.
+template <int N, int PI>
+__device__ __forceinline__ void fixed_item(float2 *data, uint32_t u, const float2 *tw)
+{
+   constexpr FixedPlan P   = Fixed<N>::P;
+   constexpr int       s   = P.passFirst[PI];
+   constexpr int       R1  = P.radix[s];
+   constexpr int       R2  = P.passLen[PI] == 2 ? P.radix[s + 1] : 1;
+   constexpr int       R   = R1 * R2;
+   constexpr int       M   = P.M[s];
+   constexpr int       q1  = M / R1;
+   constexpr int       q2  = q1 / R2;
+   const float2       *w1  = tw + P.twOff[s];
+   const float2       *w2  = tw + P.twOff[R2 > 1 ? s + 1 : s];
+   const uint32_t      d   = u / (uint32_t) q2;
+   const uint32_t      p2  = u - d * (uint32_t) q2;
+   const uint32_t      loc = d * (uint32_t) M + p2;
+   float2              v[R];
+#pragma unroll
+   for (int j = 0; j < R; ++j) v[j] = data[fslot(loc + (uint32_t) (q2 * j))];
+#pragma unroll
+   for (int j2 = 0; j2 < R2; ++j2) butterfly<R1, (q1 > 1)>(v, j2, R2, w1, q1, p2 + (uint32_t) (q2 * j2));
+   if constexpr (R2 > 1) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) butterfly<R2, (q2 > 1)>(v, R2 * m, 1, w2, q2, p2);
+   }
+#pragma unroll
+   for (int j = 0; j < R; ++j) data[fslot(loc + (uint32_t) (q2 * j))] = v[j];
+}
+
+// Register pass PI over one transform in LDS: work item t takes items
+// t, t + T, ... (unrolled for up to 4 items per work item).
+template <int N, int PI>
+__device__ __forceinline__ void fixed_pass(float2 *data, uint32_t t, const float2 *tw)
+{
+   constexpr FixedPlan P     = Fixed<N>::P;
+   constexpr int       s     = P.passFirst[PI];
+   constexpr int       R     = P.radix[s] * (P.passLen[PI] == 2 ? P.radix[s + 1] : 1);
+   constexpr int       items = N / R;
+   constexpr int       T     = P.T;
+   constexpr int       iters = (items + T - 1) / T;
+   if constexpr (iters <= 4) {
+#pragma unroll
+      for (int it = 0; it < iters; ++it) {
+         const uint32_t u = t + (uint32_t) (it * T);
+         if (items % T != 0 && u >= (uint32_t) items) break;
+         fixed_item<N, PI>(data, u, tw);
+      }
+   }
+   else {
+#pragma unroll 1
+      for (uint32_t u = t; u < (uint32_t) items; u += T) fixed_item<N, PI>(data, u, tw);
+   }
+}
+
+template <int N, int PI>
+__device__ __forceinline__ void fixed_passes_from(float2 *data, uint32_t t, const float2 *tw)
+{
+   constexpr FixedPlan P = Fixed<N>::P;
+   if constexpr (PI < P.nPasses) {
+      fixed_pass<N, PI>(data, t, tw);
+      __syncthreads();
+      fixed_passes_from<N, PI + 1>(data, t, tw);
+   }
+}
+
+// Scaling of the inverse transforms' input as two factors: forward (1, 1);
+// batched inverse (1/N, -1/N) = conj then scale; single-channel inverse
+// (1, -1) = conj. Multiplying by +-1 or by the scale is exact or identical
+// to the separate conj and scale steps.
+struct InScale {
+   float sx, sy;
+};
+__device__ __forceinline__ InScale in_scale(int mode, float scale)
+{
+   if (mode == C2C_INVERSE_PRESCALE) return InScale{scale, -scale};
+   if (mode == C2C_INVERSE_POSTDIV) return InScale{1.0f, -1.0f};
+   return InScale{1.0f, 1.0f};
+}
+
+// Everything a kernel needs besides the LDS data: transform kind and mode,
+// the caller's buffers and the per-channel strides (in complex points).
+struct FixedArgs {
+   const float2 *x;
+   float2       *y;
+   const float2 *tw;
+   const float  *sf;          // split factors (real transforms)
+   uint32_t      numChannels;
+   uint32_t      numGroups;
+   int           mode;        // C2CMode
+   uint32_t      aux;         // R2C outmode / C2R inMode
+   uint32_t      inStride;    // complex points per input channel
+   uint32_t      outStride;   // complex points per output channel
+   const uint64_t *chanOff;   // ragged batches: complex offset of each channel in x and y (else null)
+};
+
+// First complex element of channel ch in the input and in the output.
+__device__ __forceinline__ size_t in_base(const FixedArgs &a, uint32_t ch)
+{
+   return a.chanOff ? (size_t) a.chanOff[ch] : (size_t) ch * a.inStride;
+}
+__device__ __forceinline__ size_t out_base(const FixedArgs &a, uint32_t ch)
+{
+   return a.chanOff ? (size_t) a.chanOff[ch] : (size_t) ch * a.outStride;
+}
+
+// Point i (< N) of the complex sequence the FFT of channel `in` transforms:
+// the input itself (C2C, R2C), or the C2R pre-split
+// Z[i] = X[i] A'[i] + V[i] B'[i], V = conj(X[N-i]) (inMode 0) or X[N+i]
+// (inMode 1), A' = sf[2i..], B' = sf[2N+2i..] (2N reals per channel).
+template <int N, int KIND>
+__device__ __forceinline__ float2 fixed_input(const float2 *in, uint32_t i, const FixedArgs &a)
+{
+   if constexpr (KIND != KIND_C2R) {
+      return in[i];
+   }
+   else {
+      const float2 v0 = in[i];
+      float2       v1;
+      if (a.aux) {
+         v1 = in[N + i];
+      }
+      else {
+         v1   = in[N - i];
+         v1.y = -v1.y;
+      }
+      const float2 A = make_float2(a.sf[2u * i], a.sf[2u * i + 1u]);
+      const float2 B = make_float2(a.sf[2u * N + 2u * i], a.sf[2u * N + 2u * i + 1u]);
+      return make_float2(v0.x * A.x - v0.y * A.y + (v1.x * B.x - v1.y * B.y),
+                         v0.x * A.y + v0.y * A.x + (v1.x * B.y + v1.y * B.x));
+   }
+}
+
+// First pass split in two halves so that the HBM reads of the next group of
+// transforms can be issued before the current group's LDS passes: the loads
+// of pass 0 (work item u reads points u + q2*j of its transform; pass 0 has
+// d = 0) and the butterflies + LDS stores of pass 0.
+template <int N>
+struct Pass0 {
+   static constexpr FixedPlan P     = Fixed<N>::P;
+   static constexpr int       s     = P.passFirst[0];
+   static constexpr int       R1    = P.radix[s];
+   static constexpr int       R2    = P.passLen[0] == 2 ? P.radix[s + 1] : 1;
+   static constexpr int       R     = R1 * R2;
+   static constexpr int       q2    = N / R;
+   static constexpr int       items = N / R;
+   static constexpr int       iters = (items + P.T - 1) / P.T;
+};
+
+// Loads are unconditional so that they issue back to back: work items of a
+// transform beyond the batch re-read channel 0 of the group (their results
+// are never stored); items beyond the pass's item count re-read the last item.
+template <int N, int KIND>
+__device__ __forceinline__ void pass0_load_one(float2 (&v)[Pass0<N>::R], uint32_t first, uint32_t fl, uint32_t nf,
+                                               uint32_t u, const FixedArgs &a)
+{
+   using Q           = Pass0<N>;
+   const float2 *in  = a.x + in_base(a, first + (fl < nf ? fl : 0u));
+   if (Q::items % Q::P.T != 0) u = min(u, (uint32_t) Q::items - 1u);
+#pragma unroll
+   for (int j = 0; j < Q::R; ++j) v[j] = fixed_input<N, KIND>(in, u + (uint32_t) (Q::q2 * j), a);
+}
+
+template <int N>
+__device__ __forceinline__ void pass0_compute_one(float2 (&w)[Pass0<N>::R], float2 *data, uint32_t u,
+                                                  const float2 *tw, InScale sc)
+{
+   using Q          = Pass0<N>;
+   constexpr int q1 = N / Q::R1;
+   if (Q::items % Q::P.T != 0 && u >= (uint32_t) Q::items) return;
+   const float2 *w1 = tw + Q::P.twOff[Q::s];
+   const float2 *w2 = tw + Q::P.twOff[Q::R2 > 1 ? Q::s + 1 : Q::s];
+#pragma unroll
+   for (int j = 0; j < Q::R; ++j) {
+      w[j].x = sc.sx * w[j].x;
+      w[j].y = sc.sy * w[j].y;
+   }
+#pragma unroll
+   for (int j2 = 0; j2 < Q::R2; ++j2) butterfly<Q::R1, (q1 > 1)>(w, j2, Q::R2, w1, q1, u + (uint32_t) (Q::q2 * j2));
+   if constexpr (Q::R2 > 1) {
+#pragma unroll
+      for (int m = 0; m < Q::R1; ++m) butterfly<Q::R2, (Q::q2 > 1)>(w, Q::R2 * m, 1, w2, Q::q2, u);
+   }
+#pragma unroll
+   for (int j = 0; j < Q::R; ++j) data[fslot(u + (uint32_t) (Q::q2 * j))] = w[j];
+}
+
+template <int N, int KIND>
+__device__ __forceinline__ void pass0_load(float2 (&v)[Pass0<N>::iters][Pass0<N>::R], uint32_t first, uint32_t fl,
+                                           uint32_t nf, uint32_t t, const FixedArgs &a)
+{
+#pragma unroll
+   for (int it = 0; it < Pass0<N>::iters; ++it)
+      pass0_load_one<N, KIND>(v[it], first, fl, nf, t + (uint32_t) (it * Pass0<N>::P.T), a);
+}
+
+template <int N>
+__device__ __forceinline__ void pass0_compute(float2 (&v)[Pass0<N>::iters][Pass0<N>::R], float2 *data, uint32_t t,
+                                              const float2 *tw, InScale sc)
+{
+#pragma unroll
+   for (int it = 0; it < Pass0<N>::iters; ++it)
+      pass0_compute_one<N>(v[it], data, t + (uint32_t) (it * Pass0<N>::P.T), tw, sc);
+}
+
+// Output k of the complex FFT in LDS after the single-channel inverse's
+// division (the other modes are a sign factor applied by the caller).
+__device__ __forceinline__ float2 out_postdiv(float2 v, float n) { return make_float2(v.x / n, -(v.y / n)); }
+
+// LDS slots the store stage of work item t reads: outputs k = t + T*it of
+// its transform (and, for R2C, the mirrored N - k). They are the same for
+// every group of a persistent workgroup, so they are computed once when they
+// fit in a few registers (at most 16 outputs per work item).
+template <int N, int KIND>
+struct StoreMap {
+   static constexpr FixedPlan P      = Fixed<N>::P;
+   static constexpr int       iters  = (N + P.T - 1) / P.T;
+   static constexpr bool      cached = iters <= 16;
+   static constexpr int       slots  = cached ? iters : 1;
+   uint32_t                   off[slots];
+   uint32_t                   mir[KIND == KIND_R2C ? slots : 1];
+
+   __device__ __forceinline__ void init(uint32_t t)
+   {
+      if constexpr (cached) {
+#pragma unroll
+         for (int it = 0; it < iters; ++it) {
+            const uint32_t k = min(t + (uint32_t) (it * P.T), (uint32_t) N - 1u);
+            off[it]          = fslot(fixed_position<N>(k));
+            if constexpr (KIND == KIND_R2C) mir[it] = fslot(fixed_position<N>(k == 0u ? 0u : (uint32_t) N - k));
+         }
+      }
+   }
+   __device__ __forceinline__ uint32_t at(int it, uint32_t t) const
+   {
+      if constexpr (cached) return off[it];
+      else return fslot(fixed_position<N>(min(t + (uint32_t) (it * P.T), (uint32_t) N - 1u)));
+   }
+   __device__ __forceinline__ uint32_t mirror_at(int it, uint32_t t) const
+   {
+      if constexpr (cached && KIND == KIND_R2C) return mir[it];
+      else {
+         const uint32_t k = min(t + (uint32_t) (it * P.T), (uint32_t) N - 1u);
+         return fslot(fixed_position<N>(k == 0u ? 0u : (uint32_t) N - k));
+      }
+   }
+};
+
+// Writes transform `ch` from its LDS slots `src`: work item t handles outputs
+// t, t + T, ... in chunks of up to 16 (8 for R2C; all LDS reads of a chunk
+// are issued before its stores). C2C and C2R write the N complex values in natural
+// order; R2C applies the split Y[i] = Z[i] A[i] + (Z[N-i] mirrored) B[i] for
+// i < N, the Nyquist bin Y[N] = Z0re - Z0im, and for outmode 1 the conjugate
+// upper half Y[2N-i] = conj(Y[i]).
+template <int N, int KIND>
+__device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint32_t t, const FixedArgs &a,
+                                            const StoreMap<N, KIND> &map)
+{
+   constexpr FixedPlan P     = Fixed<N>::P;
+   constexpr int       T     = P.T;
+   constexpr int       iters = StoreMap<N, KIND>::iters;
+   constexpr int       MAXC  = KIND == KIND_R2C ? 8 : 16;
+   constexpr int       CH    = iters < MAXC ? iters : MAXC;
+   constexpr bool      EXACT = N % T == 0 && iters % CH == 0;
+   float2             *out   = a.y + out_base(a, ch);
+   const float         sy    = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
+   const bool          pdiv  = a.mode == C2C_INVERSE_POSTDIV;
+#pragma unroll 1
+   for (int c0 = 0; c0 < iters; c0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) v[c] = src[map.at(c0 + c, t)];
+      if constexpr (KIND == KIND_R2C) {
+         float2 vm[CH];
+#pragma unroll
+         for (int c = 0; c < CH; ++c) vm[c] = src[map.mirror_at(c0 + c, t)];
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t i = t + (uint32_t) ((c0 + c) * T);
+            if (!EXACT && (c0 + c >= iters || i >= (uint32_t) N)) break;
+            const float2 zi = v[c], zm = vm[c];
+            const float2 A  = make_float2(a.sf[2u * i], a.sf[2u * i + 1u]);
+            const float2 B  = make_float2(a.sf[4u * N - 2u * i - 2u], a.sf[4u * N - 2u * i - 1u]);
+            float2       t1;
+            if (i == 0u) t1 = make_float2(zi.x * B.x + zi.y * B.y, zi.x * B.x - zi.y * B.y);
+            else t1 = make_float2(zm.x * B.x + zm.y * B.y, zm.x * B.y - zm.y * B.x);
+            const float2 y = make_float2(zi.x * A.x - zi.y * A.y + t1.x, zi.x * A.y + zi.y * A.x + t1.y);
+            out[i]         = y;
+            if (a.aux && i > 0u) out[2u * N - i] = make_float2(y.x, -y.y);
+            if (i == 0u) out[N] = make_float2(zi.x - zi.y, 0.0f);
+         }
+      }
+      else if (pdiv) {
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t k = t + (uint32_t) ((c0 + c) * T);
+            if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
+            out[k] = out_postdiv(v[c], (float) N);
+         }
+      }
+      else {
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t k = t + (uint32_t) ((c0 + c) * T);
+            if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
+            out[k] = make_float2(v[c].x, sy * v[c].y);
+         }
+      }
+   }
+}
+
+constexpr inline int largest_divisor_upto(int n, int cap)
+{
+   int c = n < cap ? n : cap;
+   while (n % c != 0) --c;
+   return c;
+}
+
+template <int N>
+constexpr bool fixed_direct()
+{
+   return Pass0<N>::q2 >= 32;
+}
+
+// Registers held across the LDS passes by the next group's first-pass inputs
+// are bounded: lengths whose first pass needs more than 16 points per work
+// item load them right before use instead. The C2R pre-split combines four
+// loads per point as they arrive, so C2R does not prefetch.
+template <int N, int KIND>
+constexpr bool fixed_prefetch()
+{
+   return KIND != KIND_C2R && fixed_direct<N>() && Pass0<N>::iters * Pass0<N>::R <= 16;
+}
+
+// The twiddle table is staged in LDS (once per workgroup) when it fits next
+// to the data: vector loads of twiddles from L2 would otherwise queue behind
+// the prefetched input loads (vmcnt completes in order).
+template <int N>
+constexpr bool fixed_tw_lds()
+{
+   constexpr FixedPlan P = Fixed<N>::P;
+   return ((size_t) P.FPB * fixed_stride<N>() + (size_t) P.twCount) * sizeof(float2) <= 163840u;
+}
+
+template <int N>
+constexpr size_t fixed_lds_bytes()
+{
+   constexpr FixedPlan P = Fixed<N>::P;
+   return ((size_t) P.FPB * fixed_stride<N>() + (fixed_tw_lds<N>() ? (size_t) P.twCount : 0u)) * sizeof(float2);
+}
+
+template <int N, int KIND>
+__global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
+{
+   constexpr FixedPlan P      = Fixed<N>::P;
+   constexpr int       T      = P.T;
+   constexpr int       FPB    = P.FPB;
+   constexpr uint32_t  STRIDE = fixed_stride<N>();
+   extern __shared__ float2 smem[];
+   const uint32_t fl   = threadIdx.x / T;
+   const uint32_t t    = threadIdx.x % T;
+   const InScale  sc   = in_scale(a.mode, 1.0f / (float) N);
+   float2        *data = smem + fl * STRIDE;
+   const float2  *twp  = fixed_tw_lds<N>() ? (const float2 *) (smem + FPB * STRIDE) : a.tw;
+   StoreMap<N, KIND> map;
+   map.init(t);
+
+   if constexpr (fixed_prefetch<N, KIND>()) {
+      using Q = Pass0<N>;
+      float2   v[Q::iters][Q::R];
+      uint32_t g = blockIdx.x;
+      if (g < a.numGroups) pass0_load<N, KIND>(v, g * FPB, fl, min((uint32_t) FPB, a.numChannels - g * FPB), t, a);
+      if constexpr (fixed_tw_lds<N>()) {
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+         __syncthreads();
+      }
+      for (; g < a.numGroups; g += gridDim.x) {
+         const uint32_t first = g * FPB;
+         const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+         pass0_compute<N>(v, data, t, twp, sc);
+         const uint32_t gn = g + gridDim.x;
+         if (gn < a.numGroups) pass0_load<N, KIND>(v, gn * FPB, fl, min((uint32_t) FPB, a.numChannels - gn * FPB), t, a);
+         __syncthreads();
+         fixed_passes_from<N, 1>(data, t, twp);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         __syncthreads();
+      }
+   }
+   else if constexpr (KIND == KIND_C2R && N % T == 0 && N >= 256) {
+      // C2R: work item t computes the pre-split points t + T*it of its
+      // transform from coalesced (and mirrored) reads, 8 at a time, into LDS;
+      // then every pass runs from LDS.
+      if constexpr (fixed_tw_lds<N>()) {
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+      }
+      constexpr int iters = N / T;
+      constexpr int CH    = largest_divisor_upto(iters, 8);
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+         const uint32_t first = g * FPB;
+         const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+         if (fl < nf) {
+            const float2 *in = a.x + in_base(a, first + fl);
+#pragma unroll 1
+            for (int c0 = 0; c0 < iters; c0 += CH) {
+               float2 z[CH];
+#pragma unroll
+               for (int c = 0; c < CH; ++c) z[c] = fixed_input<N, KIND>(in, t + (uint32_t) ((c0 + c) * T), a);
+#pragma unroll
+               for (int c = 0; c < CH; ++c)
+                  data[fslot(t + (uint32_t) ((c0 + c) * T))] = make_float2(sc.sx * z[c].x, sc.sy * z[c].y);
+            }
+         }
+         __syncthreads();
+         fixed_passes_from<N, 0>(data, t, twp);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         __syncthreads();
+      }
+   }
+   else if constexpr (fixed_direct<N>()) {
+      using Q = Pass0<N>;
+      if constexpr (fixed_tw_lds<N>()) {
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+         __syncthreads();
+      }
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+         const uint32_t first = g * FPB;
+         const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+#pragma unroll 1
+         for (uint32_t u0 = 0; u0 < (uint32_t) Q::items; u0 += T) {
+            float2 v[Q::R];
+            pass0_load_one<N, KIND>(v, first, fl, nf, t + u0, a);
+            pass0_compute_one<N>(v, data, t + u0, twp, sc);
+         }
+         __syncthreads();
+         fixed_passes_from<N, 1>(data, t, twp);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         __syncthreads();
+      }
+   }
+   else {
+      if constexpr (fixed_tw_lds<N>()) {
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+      }
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+         const uint32_t first = g * FPB;
+         const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+         for (uint32_t i = threadIdx.x; i < nf * N; i += kFixedThreads) {
+            const uint32_t f = i / (uint32_t) N;
+            const uint32_t k = i - f * (uint32_t) N;
+            const float2   z = fixed_input<N, KIND>(a.x + in_base(a, first + f), k, a);
+            smem[f * STRIDE + fslot(k)] = make_float2(sc.sx * z.x, sc.sy * z.y);
+         }
+         __syncthreads();
+         fixed_passes_from<N, 0>(data, t, twp);
+         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         __syncthreads();
+      }
+   }
+}
+
+// Workgroups resident at once on the device (queried once per kernel), used
+// to size the persistent grid.
+template <int N, int KIND>
+uint32_t fixed_resident_groups(size_t lds)
+{
+   static uint32_t cached = 0;
+   if (cached == 0) {
+      int dev = 0, cus = 0, per = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+         cus = 256;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fixed_kernel<N, KIND>, kFixedThreads, lds) !=
+              hipSuccess ||
+          per <= 0)
+         per = 1;
+      cached = (uint32_t) cus * (uint32_t) per;
+   }
+   return cached;
+}
+
+template <int N, int KIND>
+hipError_t launch_fixed(FixedArgs a, hipStream_t stream, uint32_t maxGrid = 0)
+{
+   constexpr FixedPlan P   = Fixed<N>::P;
+   const size_t        lds = fixed_lds_bytes<N>();
+   static bool         ldsSet = false;   // set once, so launches can be captured in a graph
+   if (lds > 65536 && !ldsSet) {
+      const hipError_t e = hipFuncSetAttribute((const void *) fixed_kernel<N, KIND>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+      if (e != hipSuccess) return e;
+      ldsSet = true;
+   }
+   a.numGroups   = (a.numChannels + P.FPB - 1) / P.FPB;
+   uint32_t grid = min(a.numGroups, fixed_resident_groups<N, KIND>(lds));
+   if (maxGrid != 0 && grid > maxGrid) grid = maxGrid;
+   hipLaunchKernelGGL((fixed_kernel<N, KIND>), dim3(grid), dim3(kFixedThreads), lds, stream, a);
+   return hipGetLastError();
+}
+
+}  // namespace fixedk
+}  // namespace fftlib

@@ -63,9 +63,11 @@ bool fits_lds(const StageTable &st);
 
 // Complex FFTs longer than one workgroup's LDS (fft_large.hip): outer radix
 // passes over HBM into `scratch` (st.n * numChannels complex values), then
-// sub-blocks finished in LDS. Same modes as launch_c2c.
+// sub-blocks finished in LDS. Same modes as launch_c2c. chanOff (optional,
+// device array) gives each channel's complex offset in x and y (ragged
+// batches); scratch stays contiguous.
 hipError_t launch_c2c_large(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
-                            int mode, float *scratch, hipStream_t stream);
+                            int mode, float *scratch, hipStream_t stream, const uint64_t *chanOff = nullptr);
 
 // Lengths with a compile-time specialised complex kernel (fft_fixed.hip).
 bool fixed_c2c_supported(uint32_t n);
@@ -74,6 +76,11 @@ bool fixed_c2c_supported(uint32_t n);
 // fixed_c2c_supported(n)); same semantics as launch_c2c.
 hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t n, uint32_t numChannels, int mode,
                             hipStream_t stream);
+
+// Ragged form: `count` transforms of n points at complex offsets chanOff[i]
+// (a device array) of x and y. maxGrid > 0 caps the persistent grid.
+hipError_t launch_c2c_fixed_ragged(const float *x, float *y, const float *tw, uint32_t n, uint32_t count, int mode,
+                                   const uint64_t *chanOff, uint32_t maxGrid, hipStream_t stream);
 
 // Inner complex lengths nc (= N/2) with specialised real-transform kernels.
 bool fixed_real_supported(uint32_t nc);

@@ -46,7 +46,8 @@ __device__ __forceinline__ uint32_t dif_inverse(uint32_t P, const StageTable &st
 template <int R1, int R2, int MODE, bool FIRST>
 __global__ void __launch_bounds__(kThreadsL) outer_pass_kernel(const float2 *src, float2 *dst,
                                                                const float2 *__restrict__ tw, StageTable st,
-                                                               uint32_t s, uint32_t numChannels)
+                                                               uint32_t s, uint32_t numChannels,
+                                                               const uint64_t *chanOff)
 {
    constexpr int  R     = R1 * R2;
    const uint32_t N     = st.n;
@@ -64,11 +65,12 @@ __global__ void __launch_bounds__(kThreadsL) outer_pass_kernel(const float2 *src
    const float2  *w2    = tw + st.twOff[R2 > 1 ? s + 1 : s];
    const float    scale = 1.0f / (float) N;
    for (uint32_t f = blockIdx.y; f < numChannels; f += gridDim.y) {
-      const size_t base = (size_t) f * N + loc;
+      const size_t base  = (size_t) f * N + loc;
+      const size_t sbase = (FIRST && chanOff) ? (size_t) chanOff[f] + loc : base;
       float2       v[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-         float2 a = src[base + (size_t) q2 * j];
+         float2 a = src[sbase + (size_t) q2 * j];
          if (FIRST && MODE != C2C_FORWARD) a.y = -a.y;
          if (FIRST && MODE == C2C_INVERSE_PRESCALE) {
             a.x = scale * a.x;
@@ -93,7 +95,8 @@ template <int MODE>
 __global__ void __launch_bounds__(kThreadsL) inner_block_kernel(const float2 *__restrict__ t,
                                                                 float2 *__restrict__ y,
                                                                 const float2 *__restrict__ tw, StageTable st,
-                                                                uint32_t p0, uint32_t M2, uint32_t numChannels)
+                                                                uint32_t p0, uint32_t M2, uint32_t numChannels,
+                                                                const uint64_t *chanOff)
 {
    extern __shared__ float2 smem[];
    const uint32_t N       = st.n;
@@ -121,13 +124,13 @@ __global__ void __launch_bounds__(kThreadsL) inner_block_kernel(const float2 *__
          v.x = v.x / (float) N;
          v.y = -(v.y / (float) N);
       }
-      y[(size_t) f * N + k] = v;
+      y[(chanOff ? (size_t) chanOff[f] : (size_t) f * N) + k] = v;
    }
 }
 
 template <int MODE, bool FIRST>
 hipError_t launch_outer(const float2 *src, float2 *dst, const float2 *tw, const StageTable &st, uint32_t pass,
-                        uint32_t numChannels, hipStream_t stream)
+                        uint32_t numChannels, const uint64_t *chanOff, hipStream_t stream)
 {
    const uint32_t s  = st.passFirst[pass];
    const uint32_t r1 = st.radix[s];
@@ -138,7 +141,7 @@ hipError_t launch_outer(const float2 *src, float2 *dst, const float2 *tw, const 
 #define FFTLIB_OUTER(A, B)                                                                                          \
    case A * 8 + B:                                                                                                   \
       hipLaunchKernelGGL((outer_pass_kernel<A, B, MODE, FIRST>), grid, block, 0, stream, src, dst, tw, st, s,       \
-                         numChannels);                                                                               \
+                         numChannels, chanOff);                                                                      \
       break;
    switch (r1 * 8 + r2) {
       FFTLIB_OUTER(5, 5)
@@ -161,25 +164,27 @@ hipError_t launch_outer(const float2 *src, float2 *dst, const float2 *tw, const 
 
 template <int MODE>
 hipError_t run_large(const float2 *x, float2 *y, const float2 *tw, const StageTable &st, uint32_t numChannels,
-                     float2 *scratch, uint32_t p0, hipStream_t stream)
+                     float2 *scratch, uint32_t p0, const uint64_t *chanOff, hipStream_t stream)
 {
    hipError_t e;
    for (uint32_t p = 0; p < p0; ++p) {
-      e = (p == 0) ? launch_outer<MODE, true>(x, scratch, tw, st, p, numChannels, stream)
-                   : launch_outer<MODE, false>(scratch, scratch, tw, st, p, numChannels, stream);
+      e = (p == 0) ? launch_outer<MODE, true>(x, scratch, tw, st, p, numChannels, chanOff, stream)
+                   : launch_outer<MODE, false>(scratch, scratch, tw, st, p, numChannels, nullptr, stream);
       if (e != hipSuccess) return e;
    }
    const uint32_t M2      = st.stageM[st.passFirst[p0]];
    const size_t   lds     = (size_t) lds_points(1, M2) * sizeof(float2);
    const uint32_t total   = numChannels * (st.n / M2);
    const uint32_t per     = (total + kXcds - 1) / kXcds;
-   if (lds > 65536) {
+   static bool ldsSet = false;   // largest sub-block's LDS, set once (graph capture safe)
+   if (!ldsSet) {
       e = hipFuncSetAttribute((const void *) inner_block_kernel<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int) lds);
+                              (int) (lds_points(1, kInnerPoints) * sizeof(float2)));
       if (e != hipSuccess) return e;
+      ldsSet = true;
    }
    hipLaunchKernelGGL(inner_block_kernel<MODE>, dim3(per * kXcds), dim3(kThreadsL), lds, stream,
-                      (const float2 *) scratch, y, tw, st, p0, M2, numChannels);
+                      (const float2 *) scratch, y, tw, st, p0, M2, numChannels, chanOff);
    return hipGetLastError();
 }
 
@@ -193,17 +198,17 @@ uint32_t large_first_inner_pass(const StageTable &st)
 }
 
 hipError_t launch_c2c_large(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
-                            int mode, float *scratch, hipStream_t stream)
+                            int mode, float *scratch, hipStream_t stream, const uint64_t *chanOff)
 {
    if (numChannels == 0) return hipSuccess;
    const uint32_t p0 = large_first_inner_pass(st);
    if (p0 == 0) return hipErrorInvalidValue;
    const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
    float2       *yo = (float2 *) y, *t = (float2 *) scratch;
-   if (mode == C2C_FORWARD) return run_large<C2C_FORWARD>(xi, yo, wi, st, numChannels, t, p0, stream);
+   if (mode == C2C_FORWARD) return run_large<C2C_FORWARD>(xi, yo, wi, st, numChannels, t, p0, chanOff, stream);
    if (mode == C2C_INVERSE_PRESCALE)
-      return run_large<C2C_INVERSE_PRESCALE>(xi, yo, wi, st, numChannels, t, p0, stream);
-   return run_large<C2C_INVERSE_POSTDIV>(xi, yo, wi, st, numChannels, t, p0, stream);
+      return run_large<C2C_INVERSE_PRESCALE>(xi, yo, wi, st, numChannels, t, p0, chanOff, stream);
+   return run_large<C2C_INVERSE_POSTDIV>(xi, yo, wi, st, numChannels, t, p0, chanOff, stream);
 }
 
 }  // namespace fftlib

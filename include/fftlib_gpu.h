@@ -58,6 +58,41 @@ FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable);
 FFTLIB_STATUS FFTLIB_GPU_setC2CVariant(int32_t variant);
 int32_t       FFTLIB_GPU_getC2CVariant(void);
 
+/* Ragged batches (no counterpart in the reference, whose batched kernels take
+ * one length per call: a reference caller issues one _kernel per length).
+ *
+ * FFTLIB_GPU_raggedCreate plans `count` complex transforms of lengths[i]
+ * points (2/3/5-smooth, 16 <= lengths[i] <= 2^20) stored back to back:
+ * transform i starts at complex element sum_{j<i} lengths[j] of every buffer
+ * passed to FFTLIB_GPU_raggedExec. Each transform has the C2C semantics of
+ * FFTLIB_fft1dBatched_i32fc_c32fc_o32fc (forward) or
+ * FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc (inverse) with the library's own
+ * twiddle tables.
+ *
+ * FFTLIB_GPU_raggedExec runs the whole batch as one HIP graph launch on the
+ * library stream (captured on the first call and again whenever the buffers
+ * or the direction change): direction 0 forward pX -> pY, 1 inverse
+ * pX -> pY, 2 forward pX -> pY then inverse pY -> pZ. Buffers are device
+ * memory, 8-byte aligned; pX is not written. Calls on one handle are
+ * serialised; calls must not overlap with other calls that use the same
+ * handle's buffers.
+ *
+ * FFTLIB_GPU_raggedTune times the candidate persistent-grid sizes of every
+ * group (one length) with the caller's buffers (pY is overwritten) and keeps
+ * the fastest in a process-wide cache that later plans of the same lengths
+ * and group sizes use; FFTLIB_GPU_tuneSave / FFTLIB_GPU_tuneLoad write and
+ * read that cache as text ("fftlib-amd-tune 1", then one
+ * "length group-size-bucket grid-cap microseconds" line per entry). */
+typedef struct FFTLIB_GPU_ragged_s *FFTLIB_GPU_raggedHandle;
+FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uint32_t *lengths, uint32_t count);
+FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB_F32 *pX, FFTLIB_F32 *pY,
+                                    FFTLIB_F32 *pZ, int32_t direction);
+FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB_F32 *pX, FFTLIB_F32 *pY);
+FFTLIB_STATUS FFTLIB_GPU_raggedDestroy(FFTLIB_GPU_raggedHandle handle);
+uint64_t      FFTLIB_GPU_raggedPoints(FFTLIB_GPU_raggedHandle handle);
+FFTLIB_STATUS FFTLIB_GPU_tuneSave(const char *path);
+FFTLIB_STATUS FFTLIB_GPU_tuneLoad(const char *path);
+
 /* Waits for all work the library enqueued on its stream. */
 FFTLIB_STATUS FFTLIB_GPU_synchronize(void);
 
