@@ -160,8 +160,26 @@ hipError_t launch_c2c_fixed_ragged(const float *x, float *y, const float *tw, ui
    if (count == 0) return hipSuccess;
    FixedArgs a{};
    a.x = (const float2 *) x, a.y = (float2 *) y, a.tw = (const float2 *) tw, a.sf = nullptr;
-   a.numChannels = count, a.mode = mode, a.aux = 0, a.inStride = n, a.outStride = n, a.chanOff = chanOff;
+   a.numChannels = count, a.mode = mode, a.aux = 0, a.inStride = n, a.outStride = n;
+   a.inOff = chanOff, a.outOff = chanOff;
    return dispatch_c2c_any(n, a, maxGrid, stream);
+}
+
+hipError_t launch_c2c_fixed_blocks(const float *scratch, float *y, const float *twTail, uint32_t m2,
+                                   uint32_t numTransforms, const StageTable &full, const uint64_t *outOff, int mode,
+                                   hipStream_t stream)
+{
+   if (numTransforms == 0) return hipSuccess;
+   if (full.n % m2 != 0 || full.nDigits > 24) return hipErrorInvalidValue;
+   FixedArgs a{};
+   a.x = (const float2 *) scratch, a.y = (float2 *) y, a.tw = (const float2 *) twTail, a.sf = nullptr;
+   a.nBlocks     = full.n / m2;
+   a.numChannels = numTransforms * a.nBlocks;
+   a.mode = mode, a.aux = 0, a.inStride = m2, a.outStride = 0;
+   a.inOff = nullptr, a.outOff = outOff;
+   a.fullN = full.n, a.fullPow2 = full.pow2, a.fullLog2 = full.log2n, a.fullDigits = full.nDigits;
+   for (uint32_t i = 0; i < full.nDigits; ++i) a.fullDigit[i] = full.digit[i];
+   return dispatch_c2c_any(m2, a, 0, stream);
 }
 
 hipError_t launch_r2c_fixed(const float *x, float *y, const float *tw, const float *sf, uint32_t nc,

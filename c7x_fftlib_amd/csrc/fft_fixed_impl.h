@@ -211,17 +211,60 @@ struct FixedArgs {
    uint32_t      aux;         // R2C outmode / C2R inMode
    uint32_t      inStride;    // complex points per input channel
    uint32_t      outStride;   // complex points per output channel
-   const uint64_t *chanOff;   // ragged batches: complex offset of each channel in x and y (else null)
+   const uint64_t *inOff;     // ragged batches: complex offset of each channel in x (else null)
+   const uint64_t *outOff;    // ... and in y (of each full transform in block mode)
+   // Block mode (the LDS stage of transforms longer than one workgroup):
+   // channel g is sub-block b = g % nBlocks of transform f = g / nBlocks,
+   // read contiguously from x; its output kk goes to element
+   // c_b + nBlocks*kk of transform f in y, where c_b is the frequency whose
+   // in-place DIF position of the full transform is b*N (N = this kernel's
+   // length). The full transform's digit sequence gives c_b.
+   uint32_t nBlocks;          // 0 = not block mode
+   uint32_t fullN;
+   uint32_t fullPow2, fullLog2, fullDigits;
+   uint8_t  fullDigit[24];
 };
+
+// Frequency of the full transform whose in-place DIF position is P.
+__device__ __forceinline__ uint32_t full_dif_inverse(const FixedArgs &a, uint32_t P)
+{
+   if (a.fullPow2) return a.fullLog2 ? (__brev(P) >> (32u - a.fullLog2)) : 0u;
+   uint32_t k = 0;
+   for (int i = (int) a.fullDigits - 1; i >= 0; --i) {
+      const uint32_t r  = a.fullDigit[i];
+      const uint32_t qd = P / r;
+      k                 = k * r + (P - qd * r);
+      P                 = qd;
+   }
+   return k;
+}
+
+// First group a persistent workgroup takes. In block mode the sub-blocks of
+// one transform (consecutive groups) go to workgroups of the same XCD
+// (dispatch places workgroup w on XCD w % 8) in the same round, so the
+// interleaved outputs of one transform meet in that XCD's L2.
+__device__ __forceinline__ uint32_t first_group(const FixedArgs &a)
+{
+   const uint32_t w = blockIdx.x, grid = gridDim.x;
+   if (a.nBlocks == 0u || grid < 8u || (grid & 7u) != 0u) return w;
+   return (w & 7u) * (grid >> 3) + (w >> 3);
+}
+
+
 
 // First complex element of channel ch in the input and in the output.
 __device__ __forceinline__ size_t in_base(const FixedArgs &a, uint32_t ch)
 {
-   return a.chanOff ? (size_t) a.chanOff[ch] : (size_t) ch * a.inStride;
+   return a.inOff ? (size_t) a.inOff[ch] : (size_t) ch * a.inStride;
 }
 __device__ __forceinline__ size_t out_base(const FixedArgs &a, uint32_t ch)
 {
-   return a.chanOff ? (size_t) a.chanOff[ch] : (size_t) ch * a.outStride;
+   if (a.nBlocks) {
+      const uint32_t f = ch / a.nBlocks, b = ch - f * a.nBlocks;
+      const size_t   t = a.outOff ? (size_t) a.outOff[f] : (size_t) f * a.fullN;
+      return t + full_dif_inverse(a, b * (uint32_t) (a.fullN / a.nBlocks));
+   }
+   return a.outOff ? (size_t) a.outOff[ch] : (size_t) ch * a.outStride;
 }
 
 // Point i (< N) of the complex sequence the FFT of channel `in` transforms:
@@ -383,6 +426,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
    constexpr int       CH    = iters < MAXC ? iters : MAXC;
    constexpr bool      EXACT = N % T == 0 && iters % CH == 0;
    float2             *out   = a.y + out_base(a, ch);
+   const uint32_t      es    = a.nBlocks ? a.nBlocks : 1u;   // output element stride
    const float         sy    = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
    const bool          pdiv  = a.mode == C2C_INVERSE_POSTDIV;
 #pragma unroll 1
@@ -415,7 +459,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k] = out_postdiv(v[c], (float) N);
+            out[k * es] = out_postdiv(v[c], (float) (a.nBlocks ? a.fullN : (uint32_t) N));
          }
       }
       else {
@@ -423,7 +467,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k] = make_float2(v[c].x, sy * v[c].y);
+            out[k * es] = make_float2(v[c].x, sy * v[c].y);
          }
       }
    }
@@ -479,7 +523,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
    extern __shared__ float2 smem[];
    const uint32_t fl   = threadIdx.x / T;
    const uint32_t t    = threadIdx.x % T;
-   const InScale  sc   = in_scale(a.mode, 1.0f / (float) N);
+   const InScale  sc   = a.nBlocks ? InScale{1.0f, 1.0f} : in_scale(a.mode, 1.0f / (float) N);
    float2        *data = smem + fl * STRIDE;
    const float2  *twp  = fixed_tw_lds<N>() ? (const float2 *) (smem + FPB * STRIDE) : a.tw;
    StoreMap<N, KIND> map;
@@ -488,7 +532,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
    if constexpr (fixed_prefetch<N, KIND>()) {
       using Q = Pass0<N>;
       float2   v[Q::iters][Q::R];
-      uint32_t g = blockIdx.x;
+      uint32_t g = first_group(a);
       if (g < a.numGroups) pass0_load<N, KIND>(v, g * FPB, fl, min((uint32_t) FPB, a.numChannels - g * FPB), t, a);
       if constexpr (fixed_tw_lds<N>()) {
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
@@ -515,7 +559,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       }
       constexpr int iters = N / T;
       constexpr int CH    = largest_divisor_upto(iters, 8);
-      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+      for (uint32_t g = first_group(a); g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
          if (fl < nf) {
@@ -542,7 +586,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
          __syncthreads();
       }
-      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+      for (uint32_t g = first_group(a); g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
 #pragma unroll 1
@@ -561,7 +605,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       if constexpr (fixed_tw_lds<N>()) {
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
       }
-      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
+      for (uint32_t g = first_group(a); g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
          for (uint32_t i = threadIdx.x; i < nf * N; i += kFixedThreads) {

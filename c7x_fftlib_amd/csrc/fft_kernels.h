@@ -82,6 +82,17 @@ hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t 
 hipError_t launch_c2c_fixed_ragged(const float *x, float *y, const float *tw, uint32_t n, uint32_t count, int mode,
                                    const uint64_t *chanOff, uint32_t maxGrid, hipStream_t stream);
 
+// LDS stage of a transform longer than one workgroup's LDS on the specialised
+// kernel of the sub-block length m2 (fixed_c2c_supported(m2)): numTransforms
+// transforms of full.n points whose first passes left contiguous sub-blocks of
+// m2 points in `scratch`; twTail = the full table from the first stage of the
+// sub-blocks. Outputs go to natural order in y (outOff: ragged offsets of the
+// full transforms, or null for contiguous ones); mode as in launch_c2c, the
+// input conj/scale having been applied by the first outer pass.
+hipError_t launch_c2c_fixed_blocks(const float *scratch, float *y, const float *twTail, uint32_t m2,
+                                   uint32_t numTransforms, const StageTable &full, const uint64_t *outOff, int mode,
+                                   hipStream_t stream);
+
 // Inner complex lengths nc (= N/2) with specialised real-transform kernels.
 bool fixed_real_supported(uint32_t nc);
 

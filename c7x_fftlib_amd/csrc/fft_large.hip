@@ -173,6 +173,10 @@ hipError_t run_large(const float2 *x, float2 *y, const float2 *tw, const StageTa
       if (e != hipSuccess) return e;
    }
    const uint32_t M2      = st.stageM[st.passFirst[p0]];
+   if (fixed_c2c_supported(M2))
+      return launch_c2c_fixed_blocks((const float *) scratch, (float *) y,
+                                     (const float *) (tw + st.twOff[st.passFirst[p0]]), M2, numChannels, st, chanOff,
+                                     MODE, stream);
    const size_t   lds     = (size_t) lds_points(1, M2) * sizeof(float2);
    const uint32_t total   = numChannels * (st.n / M2);
    const uint32_t per     = (total + kXcds - 1) / kXcds;
