@@ -20,10 +20,10 @@ OBJ_DIR = os.path.join(HERE, "_build")
 SOURCES = [
     "fft_kernels.hip",
     "fft_fixed.hip",
-    "fft_fixed_ext0.hip",
-    "fft_fixed_ext1.hip",
-    "fft_fixed_ext2.hip",
-    "fft_fixed_ext3.hip",
+    "fft_fixed_class0.hip",
+    "fft_fixed_class1.hip",
+    "fft_fixed_class2.hip",
+    "fft_fixed_class3.hip",
     "fft_large.hip",
     "fft_mfma.hip",
     "fft_ragged.hip",

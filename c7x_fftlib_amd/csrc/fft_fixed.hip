@@ -35,7 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_fixed_impl.h"
-#include "fft_fixed_ext.h"
+#include "fft_fixed_class.h"
 
 namespace fftlib {
 using namespace fixedk;
@@ -62,38 +62,14 @@ hipError_t dispatch_fixed_real(uint32_t n, const FixedArgs &a, hipStream_t strea
 
 }  // namespace
 
-hipError_t fixed_ext_part0_launch(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream);
-hipError_t fixed_ext_part1_launch(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream);
-hipError_t fixed_ext_part2_launch(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream);
-hipError_t fixed_ext_part3_launch(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream);
-bool       fixed_ext_part0_has(uint32_t n);
-bool       fixed_ext_part1_has(uint32_t n);
-bool       fixed_ext_part2_has(uint32_t n);
-bool       fixed_ext_part3_has(uint32_t n);
-
-bool fixed_ext_has(int part, uint32_t n)
-{
-   switch (part) {
-   case 0: return fixed_ext_part0_has(n);
-   case 1: return fixed_ext_part1_has(n);
-   case 2: return fixed_ext_part2_has(n);
-   case 3: return fixed_ext_part3_has(n);
-   default: return false;
-   }
-}
-
-hipError_t fixed_ext_launch(int part, uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream)
-{
-   switch (part) {
-   case 0: return fixed_ext_part0_launch(n, a, maxGrid, stream);
-   case 1: return fixed_ext_part1_launch(n, a, maxGrid, stream);
-   case 2: return fixed_ext_part2_launch(n, a, maxGrid, stream);
-   case 3: return fixed_ext_part3_launch(n, a, maxGrid, stream);
-   default: return hipErrorInvalidValue;
-   }
-}
-
 namespace {
+
+bool smooth235(uint32_t n)
+{
+   for (uint32_t p : {2u, 3u, 5u})
+      while (n % p == 0u) n /= p;
+   return n == 1u;
+}
 
 bool main_c2c_has(uint32_t n)
 {
@@ -106,7 +82,56 @@ bool main_c2c_has(uint32_t n)
    }
 }
 
-// Complex FFT of length n on its specialised kernel, wherever it is instantiated.
+// Work items per transform and transforms per workgroup of the specialised
+// kernels (make_fixed_plan's rule).
+uint32_t fixed_fpb(uint32_t n)
+{
+   uint32_t T = 1;
+   while (T < 256u && T * 16u < n) T *= 2u;
+   return (uint32_t) kFixedThreads / T;
+}
+
+// One task of a class kernel covering `count` channels of length n.
+ClassTask single_task(uint32_t n, uint32_t count, const float2 *tw, const uint64_t *inOff, const uint64_t *outOff)
+{
+   ClassTask k{};
+   k.n = n, k.numChannels = count, k.tw = tw, k.inOff = inOff, k.outOff = outOff;
+   k.numGroups = (count + fixed_fpb(n) - 1u) / fixed_fpb(n);
+   return k;
+}
+
+}  // namespace
+
+int fixed_class_of(uint32_t n)
+{
+   if (n < 16u || n > 16384u || !smooth235(n)) return -1;
+   int k = 0;
+   while ((16u << k) < n) ++k;
+   return k;
+}
+
+hipError_t fixed_class_launch(int cls, const ClassArgs &c, uint32_t maxGrid, hipStream_t stream)
+{
+   switch (cls) {
+   case 0: return fixed_class_launch_tu1(cls, c, maxGrid, stream);
+   case 1: return fixed_class_launch_tu0(cls, c, maxGrid, stream);
+   case 2: return fixed_class_launch_tu2(cls, c, maxGrid, stream);
+   case 3: return fixed_class_launch_tu3(cls, c, maxGrid, stream);
+   case 4: return fixed_class_launch_tu1(cls, c, maxGrid, stream);
+   case 5: return fixed_class_launch_tu2(cls, c, maxGrid, stream);
+   case 6: return fixed_class_launch_tu3(cls, c, maxGrid, stream);
+   case 7: return fixed_class_launch_tu3(cls, c, maxGrid, stream);
+   case 8: return fixed_class_launch_tu2(cls, c, maxGrid, stream);
+   case 9: return fixed_class_launch_tu1(cls, c, maxGrid, stream);
+   case 10: return fixed_class_launch_tu0(cls, c, maxGrid, stream);
+   default: return hipErrorInvalidValue;
+   }
+}
+
+namespace {
+
+// Complex FFT of length n on its specialised kernel: the dedicated one for
+// the lengths listed above, else its class kernel with one task.
 hipError_t dispatch_c2c_any(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream)
 {
    if (main_c2c_has(n)) {
@@ -118,20 +143,20 @@ hipError_t dispatch_c2c_any(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hi
       default: break;
       }
    }
-   for (int part = 0; part < kFixedExtParts; ++part)
-      if (fixed_ext_has(part, n)) return fixed_ext_launch(part, n, a, maxGrid, stream);
-   return hipErrorInvalidValue;
+   const int cls = fixed_class_of(n);
+   if (cls < 0) return hipErrorInvalidValue;
+   ClassArgs c{};
+   c.x = a.x, c.y = a.y, c.scratch = a.x, c.mode = a.mode;
+   c.single = single_task(n, a.numChannels, a.tw, a.inOff, a.outOff);
+   c.single.nBlocks = a.nBlocks, c.single.fullN = a.fullN, c.single.fullPow2 = a.fullPow2;
+   c.single.fullLog2 = a.fullLog2, c.single.fullDigits = a.fullDigits, c.single.fullDigitCode = a.fullDigitCode;
+   c.numItems = c.single.numGroups;
+   return fixed_class_launch(cls, c, maxGrid, stream);
 }
 
 }  // namespace
 
-bool fixed_c2c_supported(uint32_t n)
-{
-   if (main_c2c_has(n)) return true;
-   for (int part = 0; part < kFixedExtParts; ++part)
-      if (fixed_ext_has(part, n)) return true;
-   return false;
-}
+bool fixed_c2c_supported(uint32_t n) { return main_c2c_has(n) || fixed_class_of(n) >= 0; }
 
 bool fixed_real_supported(uint32_t nc)
 {
@@ -170,7 +195,7 @@ hipError_t launch_c2c_fixed_blocks(const float *scratch, float *y, const float *
                                    hipStream_t stream)
 {
    if (numTransforms == 0) return hipSuccess;
-   if (full.n % m2 != 0 || full.nDigits > 24) return hipErrorInvalidValue;
+   if (full.n % m2 != 0 || full.nDigits > 32) return hipErrorInvalidValue;
    FixedArgs a{};
    a.x = (const float2 *) scratch, a.y = (float2 *) y, a.tw = (const float2 *) twTail, a.sf = nullptr;
    a.nBlocks     = full.n / m2;
@@ -178,7 +203,7 @@ hipError_t launch_c2c_fixed_blocks(const float *scratch, float *y, const float *
    a.mode = mode, a.aux = 0, a.inStride = m2, a.outStride = 0;
    a.inOff = nullptr, a.outOff = outOff;
    a.fullN = full.n, a.fullPow2 = full.pow2, a.fullLog2 = full.log2n, a.fullDigits = full.nDigits;
-   for (uint32_t i = 0; i < full.nDigits; ++i) a.fullDigit[i] = full.digit[i];
+   a.fullDigitCode = pack_digits(full.digit, full.nDigits);
    return dispatch_c2c_any(m2, a, 0, stream);
 }
 

@@ -222,8 +222,16 @@ struct FixedArgs {
    uint32_t nBlocks;          // 0 = not block mode
    uint32_t fullN;
    uint32_t fullPow2, fullLog2, fullDigits;
-   uint8_t  fullDigit[24];
+   uint64_t fullDigitCode;    // digit i (lsd first) in bits 2i..2i+1: 0 -> 2, 1 -> 3, 2 -> 5
 };
+
+// Packs a digit sequence of radices 2/3/5 (at most 32) into FixedArgs's code.
+__host__ __device__ inline uint64_t pack_digits(const uint8_t *d, uint32_t n)
+{
+   uint64_t code = 0;
+   for (uint32_t i = 0; i < n; ++i) code |= (uint64_t) (d[i] == 2 ? 0u : d[i] == 3 ? 1u : 2u) << (2u * i);
+   return code;
+}
 
 // Frequency of the full transform whose in-place DIF position is P.
 __device__ __forceinline__ uint32_t full_dif_inverse(const FixedArgs &a, uint32_t P)
@@ -231,7 +239,8 @@ __device__ __forceinline__ uint32_t full_dif_inverse(const FixedArgs &a, uint32_
    if (a.fullPow2) return a.fullLog2 ? (__brev(P) >> (32u - a.fullLog2)) : 0u;
    uint32_t k = 0;
    for (int i = (int) a.fullDigits - 1; i >= 0; --i) {
-      const uint32_t r  = a.fullDigit[i];
+      const uint32_t c  = (uint32_t) (a.fullDigitCode >> (2u * (uint32_t) i)) & 3u;
+      const uint32_t r  = c == 0u ? 2u : c == 1u ? 3u : 5u;
       const uint32_t qd = P / r;
       k                 = k * r + (P - qd * r);
       P                 = qd;
@@ -620,6 +629,179 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          __syncthreads();
       }
    }
+}
+
+// ---- multi-length ("class") kernels -------------------------------------------
+// One kernel covers every length of an octave (lengths whose work-item count T
+// and transforms per workgroup agree); it runs a list of tasks (a length, its
+// channels, twiddles, offsets), each cut into groups of FPB transforms, as a
+// persistent loop over (task, group) items. Twiddles are read from global
+// memory (L2) rather than staged in LDS, since consecutive items of one
+// workgroup may have different lengths. Ragged batches put all lengths of an
+// octave in one launch; single lengths without a dedicated kernel use it with
+// one task passed by value.
+struct ClassTask {
+   uint32_t        n;            // one of the class's lengths
+   uint32_t        numChannels;  // channels (transforms x sub-blocks in block mode)
+   uint32_t        numGroups;
+   uint32_t        fromScratch;  // block mode: input from the scratch buffer, not x
+   const float2   *tw;
+   const uint64_t *inOff;
+   const uint64_t *outOff;
+   uint64_t        scratchOff;   // complex offset of this task's scratch region
+   uint32_t        nBlocks, fullN, fullPow2, fullLog2, fullDigits;
+   uint64_t        fullDigitCode;
+};
+
+struct ClassArgs {
+   const float2    *x;
+   float2          *y;
+   const float2    *scratch;
+   int              mode;       // C2CMode
+   uint32_t         numItems;
+   const ClassTask *tasks;      // null: one task, `single`, with item = group index
+   const uint32_t  *items;      // (task << 20) | group
+   ClassTask        single;
+};
+
+constexpr uint32_t kClassGroupBits = 20;
+
+template <int N, int KIND>
+__device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, float2 *smem)
+{
+   constexpr FixedPlan P      = Fixed<N>::P;
+   constexpr int       T      = P.T;
+   constexpr int       FPB    = P.FPB;
+   constexpr uint32_t  STRIDE = fixed_stride<N>();
+   const uint32_t      fl     = threadIdx.x / T;
+   const uint32_t      t      = threadIdx.x % T;
+   const InScale       sc     = a.nBlocks ? InScale{1.0f, 1.0f} : in_scale(a.mode, 1.0f / (float) N);
+   float2             *data   = smem + fl * STRIDE;
+   const uint32_t      first  = g * FPB;
+   const uint32_t      nf     = min((uint32_t) FPB, a.numChannels - first);
+   if constexpr (fixed_direct<N>()) {
+      using Q = Pass0<N>;
+#pragma unroll 1
+      for (uint32_t u0 = 0; u0 < (uint32_t) Q::items; u0 += T) {
+         float2 v[Q::R];
+         pass0_load_one<N, KIND>(v, first, fl, nf, t + u0, a);
+         pass0_compute_one<N>(v, data, t + u0, a.tw, sc);
+      }
+      __syncthreads();
+      fixed_passes_from<N, 1>(data, t, a.tw);
+   }
+   else {
+      for (uint32_t i = threadIdx.x; i < nf * N; i += kFixedThreads) {
+         const uint32_t f = i / (uint32_t) N;
+         const uint32_t k = i - f * (uint32_t) N;
+         const float2   z = a.x[in_base(a, first + f) + k];
+         smem[f * STRIDE + fslot(k)] = make_float2(sc.sx * z.x, sc.sy * z.y);
+      }
+      __syncthreads();
+      fixed_passes_from<N, 0>(data, t, a.tw);
+   }
+   StoreMap<N, KIND> map;
+   map.init(t);
+   if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+   __syncthreads();
+}
+
+__device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const ClassTask &k)
+{
+   FixedArgs a{};
+   a.x           = k.fromScratch ? c.scratch + k.scratchOff : c.x;
+   a.y           = c.y;
+   a.tw          = k.tw;
+   a.numChannels = k.numChannels;
+   a.numGroups   = k.numGroups;
+   a.mode        = c.mode;
+   a.inStride    = k.n;
+   a.outStride   = k.n;
+   a.inOff       = k.inOff;
+   a.outOff      = k.outOff;
+   a.nBlocks     = k.nBlocks;
+   a.fullN       = k.fullN;
+   a.fullPow2    = k.fullPow2;
+   a.fullLog2    = k.fullLog2;
+   a.fullDigits    = k.fullDigits;
+   a.fullDigitCode = k.fullDigitCode;
+   return a;
+}
+
+// One group of task *tk, compiled per length as a separate function so that
+// each length gets its own register allocation (all lengths inlined into one
+// kernel body need several times the registers of the largest one). The
+// task and the kernel arguments are passed by address (kernel-argument or
+// global memory), the data through the kernel's dynamic LDS.
+template <int N>
+__device__ __attribute__((noinline)) void class_group(const ClassTask *tk, const ClassArgs *c, uint32_t g)
+{
+   extern __shared__ float2 smem_class[];
+   const FixedArgs a = class_fixed_args(*c, *tk);
+   fixed_group<N, KIND_C2C>(a, g, smem_class);
+}
+
+template <int... Ls>
+__global__ void __launch_bounds__(kFixedThreads) fixed_class_kernel(ClassArgs c)
+{
+   for (uint32_t w = blockIdx.x; w < c.numItems; w += gridDim.x) {
+      uint32_t         g  = w;
+      const ClassTask *tk = &c.single;
+      if (c.tasks) {
+         const uint32_t item = c.items[w];
+         tk                  = c.tasks + (item >> kClassGroupBits);
+         g                   = item & ((1u << kClassGroupBits) - 1u);
+      }
+      const uint32_t n = tk->n;
+      (void) ((n == (uint32_t) Ls ? (class_group<Ls>(tk, &c, g), true) : false) || ...);
+   }
+}
+
+template <int... Ls>
+constexpr size_t class_lds_bytes()
+{
+   size_t m = 0;
+   ((m = m > (size_t) Fixed<Ls>::P.FPB * fixed_stride<Ls>() * sizeof(float2)
+            ? m
+            : (size_t) Fixed<Ls>::P.FPB * fixed_stride<Ls>() * sizeof(float2)),
+    ...);
+   return m;
+}
+
+template <int... Ls>
+uint32_t class_resident_groups()
+{
+   static uint32_t cached = 0;
+   if (cached == 0) {
+      int dev = 0, cus = 0, per = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+         cus = 256;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fixed_class_kernel<Ls...>, kFixedThreads,
+                                                       class_lds_bytes<Ls...>()) != hipSuccess ||
+          per <= 0)
+         per = 1;
+      cached = (uint32_t) cus * (uint32_t) per;
+   }
+   return cached;
+}
+
+template <int... Ls>
+hipError_t launch_class(const ClassArgs &c, uint32_t maxGrid, hipStream_t stream)
+{
+   if (c.numItems == 0) return hipSuccess;
+   constexpr size_t lds    = class_lds_bytes<Ls...>();
+   static bool      ldsSet = false;
+   if (lds > 65536 && !ldsSet) {
+      const hipError_t e = hipFuncSetAttribute((const void *) fixed_class_kernel<Ls...>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+      if (e != hipSuccess) return e;
+      ldsSet = true;
+   }
+   uint32_t grid = min(c.numItems, class_resident_groups<Ls...>());
+   if (maxGrid != 0 && grid > maxGrid) grid = maxGrid;
+   hipLaunchKernelGGL((fixed_class_kernel<Ls...>), dim3(grid), dim3(kFixedThreads), lds, stream, c);
+   return hipGetLastError();
 }
 
 // Workgroups resident at once on the device (queried once per kernel), used

@@ -82,6 +82,13 @@ hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t 
 hipError_t launch_c2c_fixed_ragged(const float *x, float *y, const float *tw, uint32_t n, uint32_t count, int mode,
                                    const uint64_t *chanOff, uint32_t maxGrid, hipStream_t stream);
 
+// First pass of the sub-blocks of the large path (0 when the length has no
+// pass whose sub-length fits the LDS stage), and the outer passes alone
+// (x -> scratch), for callers that run the LDS stage themselves.
+uint32_t   large_first_inner_pass(const StageTable &st);
+hipError_t launch_c2c_large_outer(const float *x, const float *tw, const StageTable &st, uint32_t numChannels, int mode,
+                                  float *scratch, hipStream_t stream, const uint64_t *chanOff);
+
 // LDS stage of a transform longer than one workgroup's LDS on the specialised
 // kernel of the sub-block length m2 (fixed_c2c_supported(m2)): numTransforms
 // transforms of full.n points whose first passes left contiguous sub-blocks of

@@ -194,6 +194,31 @@ hipError_t run_large(const float2 *x, float2 *y, const float2 *tw, const StageTa
 
 }  // namespace
 
+uint32_t large_first_inner_pass(const StageTable &st);
+
+hipError_t launch_c2c_large_outer(const float *x, const float *tw, const StageTable &st, uint32_t numChannels, int mode,
+                                  float *scratch, hipStream_t stream, const uint64_t *chanOff)
+{
+   if (numChannels == 0) return hipSuccess;
+   const uint32_t p0 = large_first_inner_pass(st);
+   if (p0 == 0) return hipErrorInvalidValue;
+   const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
+   float2       *t  = (float2 *) scratch;
+   hipError_t    e  = hipSuccess;
+   for (uint32_t p = 0; p < p0 && e == hipSuccess; ++p) {
+      if (mode == C2C_FORWARD)
+         e = p == 0 ? launch_outer<C2C_FORWARD, true>(xi, t, wi, st, p, numChannels, chanOff, stream)
+                    : launch_outer<C2C_FORWARD, false>(t, t, wi, st, p, numChannels, nullptr, stream);
+      else if (mode == C2C_INVERSE_PRESCALE)
+         e = p == 0 ? launch_outer<C2C_INVERSE_PRESCALE, true>(xi, t, wi, st, p, numChannels, chanOff, stream)
+                    : launch_outer<C2C_INVERSE_PRESCALE, false>(t, t, wi, st, p, numChannels, nullptr, stream);
+      else
+         e = p == 0 ? launch_outer<C2C_INVERSE_POSTDIV, true>(xi, t, wi, st, p, numChannels, chanOff, stream)
+                    : launch_outer<C2C_INVERSE_POSTDIV, false>(t, t, wi, st, p, numChannels, nullptr, stream);
+   }
+   return e;
+}
+
 uint32_t large_first_inner_pass(const StageTable &st)
 {
    for (uint32_t p = 1; p < st.nPasses; ++p)

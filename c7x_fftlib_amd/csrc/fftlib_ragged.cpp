@@ -11,13 +11,16 @@
 // FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc (conj, x(1.0f/N), conj), with the
 // reference's twiddle tables.
 //
-// Plan: transforms are grouped by length. A group whose length has a
-// specialised kernel (every 2/3/5-smooth length 16..16384) runs it over the
-// group's channel offset table; longer groups take the outer-pass + LDS-block
-// path with a plan-owned scratch buffer. The groups of one direction are
-// spread over several capture streams (forked from and joined into the
-// capture origin), so the graph lets independent groups run concurrently;
-// a forward-then-inverse call joins between the two directions.
+// Plan: transforms are grouped by length. Lengths up to 16384 become tasks of
+// the multi-length kernel of their octave (fft_fixed_class.h); a longer
+// length first runs its outer radix passes into its own region of a
+// plan-owned scratch buffer, and its sub-blocks then become block-mode tasks
+// of the class kernel of the sub-block length. One direction is therefore:
+// the outer passes of the long lengths (spread over several capture streams),
+// a join, then at most one class-kernel launch per octave (again spread over
+// the streams), a join. Each class kernel walks a persistent list of
+// (task, group) items, largest tasks first, so the whole device stays busy
+// with many short transforms of different lengths at once.
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -28,6 +31,8 @@
 #include <mutex>
 #include <vector>
 
+#include "fft_fixed_class.h"
+#include "fft_fixed_impl.h"
 #include "fft_kernels.h"
 #include "fftlib_gpu.h"
 #include "fftlib_runtime.h"
@@ -43,9 +48,19 @@ struct FFTLIB_GPU_ragged_s {
       uint64_t         *dOff     = nullptr; // channel offsets (complex elements)
       float            *dTw      = nullptr; // twiddle table of length n
       uint32_t          maxGrid  = 0;       // tuned persistent grid cap (0 = resident slots)
+      size_t            scrOff   = 0;       // large groups: float offset of their scratch region
       double            work     = 0.0;     // n * log2(n) * count, for ordering
    };
    std::vector<Group> groups;
+   struct Class {
+      int                     cls      = -1;
+      uint32_t                numItems = 0;
+      fixedk::ClassTask      *dTasks   = nullptr;
+      uint32_t               *dItems   = nullptr;
+      uint32_t                maxGrid  = 0;
+      double                  work     = 0.0;
+   };
+   std::vector<Class> classes;
    uint64_t           totalPoints = 0;
    float             *scratch     = nullptr;
    size_t             scratchSize = 0;
@@ -73,7 +88,9 @@ bool smooth235(uint32_t n)
 }
 
 // ---- tuning cache -------------------------------------------------------------
-// Key: length and a power-of-two bucket of the group size.
+// Key: the class kernel (kClassKeyBase + octave index) and a power-of-two
+// bucket of its item count; value: the persistent-grid cap and its time.
+constexpr uint32_t kClassKeyBase = 100000u;
 struct TuneKey {
    uint32_t n, bucket;
    bool     operator<(const TuneKey &o) const { return n != o.n ? n < o.n : bucket < o.bucket; }
@@ -101,12 +118,19 @@ bool tuned(uint32_t n, uint32_t count, uint32_t *maxGrid)
    return true;
 }
 
-hipError_t launch_group(const FFTLIB_GPU_ragged_s::Group &g, const float *in, float *out, int mode, float *scratch,
-                        uint32_t maxGrid, hipStream_t s)
+hipError_t launch_outer(const FFTLIB_GPU_ragged_s &h, const FFTLIB_GPU_ragged_s::Group &g, const float *in, int mode,
+                        hipStream_t s)
 {
-   if (g.large)
-      return launch_c2c_large(in, out, g.dTw, g.st, g.count, mode, scratch, s, g.dOff);
-   return launch_c2c_fixed_ragged(in, out, g.dTw, g.n, g.count, mode, g.dOff, maxGrid, s);
+   return launch_c2c_large_outer(in, g.dTw, g.st, g.count, mode, h.scratch + g.scrOff, s, g.dOff);
+}
+
+hipError_t launch_class(const FFTLIB_GPU_ragged_s &h, const FFTLIB_GPU_ragged_s::Class &c, const float *in, float *out,
+                        int mode, uint32_t maxGrid, hipStream_t s)
+{
+   fixedk::ClassArgs a{};
+   a.x = (const float2 *) in, a.y = (float2 *) out, a.scratch = (const float2 *) h.scratch, a.mode = mode;
+   a.numItems = c.numItems, a.tasks = c.dTasks, a.items = c.dItems;
+   return fixed_class_launch(c.cls, a, maxGrid, s);
 }
 
 FFTLIB_STATUS fail(const char *what, hipError_t e)
@@ -123,36 +147,60 @@ void release(FFTLIB_GPU_ragged_s *h)
       if (g.dOff) (void) hipFree(g.dOff);
       if (g.dTw) (void) hipFree(g.dTw);
    }
+   for (auto &c : h->classes) {
+      if (c.dTasks) (void) hipFree(c.dTasks);
+      if (c.dItems) (void) hipFree(c.dItems);
+   }
    if (h->scratch) (void) hipFree(h->scratch);
    for (hipEvent_t e : h->events) (void) hipEventDestroy(e);
    for (hipStream_t s : h->lanes) (void) hipStreamDestroy(s);
    if (h->capture) (void) hipStreamDestroy(h->capture);
 }
 
-// Enqueues one direction: groups dealt to the lanes largest first.
-hipError_t enqueue_direction(FFTLIB_GPU_ragged_s *h, const float *in, float *out, int mode, hipStream_t origin,
-                             size_t &ev)
+// Fork from origin to every lane, and join every lane back into origin.
+hipError_t fork_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
 {
    hipError_t e;
    hipEvent_t fork = h->events[ev++];
    if ((e = hipEventRecord(fork, origin)) != hipSuccess) return e;
    for (hipStream_t lane : h->lanes)
       if ((e = hipStreamWaitEvent(lane, fork, 0)) != hipSuccess) return e;
-   // The large groups share the plan's scratch, so they run in order on lane
-   // 0; the other groups are dealt round-robin to the remaining lanes.
-   const size_t nl   = h->lanes.size();
-   size_t       next = 0;
-   for (const auto &g : h->groups) {
-      hipStream_t lane = h->lanes[0];
-      if (!g.large && nl > 1) lane = h->lanes[1 + (next++ % (nl - 1))];
-      if ((e = launch_group(g, in, out, mode, h->scratch, g.maxGrid, lane)) != hipSuccess) return e;
-   }
+   return hipSuccess;
+}
+hipError_t join_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
+{
+   hipError_t e;
    for (hipStream_t lane : h->lanes) {
       hipEvent_t join = h->events[ev++];
       if ((e = hipEventRecord(join, lane)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(origin, join, 0)) != hipSuccess) return e;
    }
    return hipSuccess;
+}
+
+// Enqueues one direction: outer passes of the long lengths, then the class
+// kernels, each phase dealt to the lanes largest first.
+hipError_t enqueue_direction(FFTLIB_GPU_ragged_s *h, const float *in, float *out, int mode, hipStream_t origin,
+                             size_t &ev)
+{
+   hipError_t e;
+   size_t     next  = 0;
+   bool       outer = false;
+   for (const auto &g : h->groups) outer = outer || g.large;
+   if (outer) {
+      if ((e = fork_lanes(h, origin, ev)) != hipSuccess) return e;
+      for (const auto &g : h->groups) {
+         if (!g.large) continue;
+         if ((e = launch_outer(*h, g, in, mode, h->lanes[next++ % h->lanes.size()])) != hipSuccess) return e;
+      }
+      if ((e = join_lanes(h, origin, ev)) != hipSuccess) return e;
+   }
+   if ((e = fork_lanes(h, origin, ev)) != hipSuccess) return e;
+   next = 0;
+   for (const auto &c : h->classes)
+      if ((e = launch_class(*h, c, in, out, mode, c.maxGrid, h->lanes[next++ % h->lanes.size()])) != hipSuccess)
+         return e;
+   return join_lanes(h, origin, ev);
 }
 
 hipError_t capture_graph(FFTLIB_GPU_ragged_s *h, const float *x, float *y, float *z, int dir)
@@ -194,6 +242,65 @@ hipError_t order_after_caller(FFTLIB_GPU_ragged_s *h)
    return e;
 }
 
+// Class-kernel tasks and items of the plan: one task per length up to 16384
+// and one block-mode task per longer length (its sub-blocks), grouped by the
+// octave of the task length; items (task << 20 | group) largest task first.
+hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
+{
+   std::map<int, std::vector<fixedk::ClassTask>> tasks;
+   std::map<int, double>                         work;
+   for (const auto &g : h->groups) {
+      fixedk::ClassTask k{};
+      if (!g.large) {
+         k.n = g.n, k.numChannels = g.count, k.tw = (const float2 *) g.dTw, k.inOff = g.dOff, k.outOff = g.dOff;
+      }
+      else {
+         const uint32_t p0 = large_first_inner_pass(g.st);
+         if (p0 == 0) return hipErrorInvalidValue;
+         const uint32_t first = g.st.passFirst[p0];
+         const uint32_t m2    = g.st.stageM[first];
+         k.n = m2, k.nBlocks = g.n / m2, k.numChannels = g.count * k.nBlocks;
+         k.fromScratch = 1, k.scratchOff = g.scrOff / 2u;
+         k.tw = (const float2 *) g.dTw + g.st.twOff[first];
+         k.inOff = nullptr, k.outOff = g.dOff;
+         k.fullN = g.n, k.fullPow2 = g.st.pow2, k.fullLog2 = g.st.log2n, k.fullDigits = g.st.nDigits;
+         k.fullDigitCode = fixedk::pack_digits(g.st.digit, g.st.nDigits);
+      }
+      const int cls = fixed_class_of(k.n);
+      if (cls < 0) return hipErrorInvalidValue;
+      uint32_t T = 1;
+      while (T < 256u && T * 16u < k.n) T *= 2u;
+      const uint32_t fpb = 256u / T;
+      k.numGroups        = (k.numChannels + fpb - 1u) / fpb;
+      tasks[cls].push_back(k);
+      work[cls] += g.work;
+   }
+   for (auto &kv : tasks) {
+      FFTLIB_GPU_ragged_s::Class c;
+      c.cls  = kv.first;
+      c.work = work[kv.first];
+      std::vector<uint32_t> items;
+      for (uint32_t t = 0; t < kv.second.size(); ++t) {
+         if (kv.second[t].numGroups >= (1u << fixedk::kClassGroupBits) || t >= 4096u) return hipErrorInvalidValue;
+         for (uint32_t grp = 0; grp < kv.second[t].numGroups; ++grp) items.push_back((t << fixedk::kClassGroupBits) | grp);
+      }
+      c.numItems = (uint32_t) items.size();
+      hipError_t e;
+      if ((e = hipMalloc((void **) &c.dTasks, kv.second.size() * sizeof(fixedk::ClassTask))) != hipSuccess) return e;
+      h->classes.push_back(c);   // owned by the plan from here on
+      if ((e = hipMemcpy(h->classes.back().dTasks, kv.second.data(), kv.second.size() * sizeof(fixedk::ClassTask),
+                         hipMemcpyHostToDevice)) != hipSuccess ||
+          (e = hipMalloc((void **) &h->classes.back().dItems, items.size() * sizeof(uint32_t))) != hipSuccess ||
+          (e = hipMemcpy(h->classes.back().dItems, items.data(), items.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice)) != hipSuccess)
+         return e;
+      (void) tuned(kClassKeyBase + (uint32_t) c.cls, c.numItems, &h->classes.back().maxGrid);
+   }
+   std::sort(h->classes.begin(), h->classes.end(),
+             [](const FFTLIB_GPU_ragged_s::Class &a, const FFTLIB_GPU_ragged_s::Class &b) { return a.work > b.work; });
+   return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -224,10 +331,13 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
          delete h;
          return FFTLIB_ERR_INVALID_DIMENSION;
       }
-      // every 2/3/5-smooth length from 16 to 16384 has a specialised kernel;
-      // longer ones take the outer-pass path and share the plan's scratch
-      g.large = !fixed_c2c_supported(g.n);
-      if (g.large) scr = std::max(scr, (size_t) g.n * g.count * 8u);
+      // lengths up to 16384 are class-kernel tasks; longer ones run outer
+      // passes into their own scratch region first
+      g.large = fixed_class_of(g.n) < 0;
+      if (g.large) {
+         g.scrOff = scr / sizeof(float);
+         scr += ((size_t) g.n * g.count * 8u + 255u) & ~(size_t) 255u;
+      }
       std::vector<float> tw(2 * (size_t) g.n);
       FFTLIB_tw_gen_c2c(tw.data(), (int32_t) g.n);
       if ((e = hipMalloc((void **) &g.dOff, kv.second.size() * sizeof(uint64_t))) != hipSuccess ||
@@ -243,11 +353,15 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       double lg = 0.0;
       for (uint32_t t = g.n; t > 1; t >>= 1) lg += 1.0;
       g.work = (double) g.n * lg * g.count;
-      (void) tuned(g.n, g.count, &g.maxGrid);
       h->groups.push_back(g);
    }
    std::sort(h->groups.begin(), h->groups.end(),
              [](const FFTLIB_GPU_ragged_s::Group &a, const FFTLIB_GPU_ragged_s::Group &b) { return a.work > b.work; });
+   if ((e = build_classes(h)) != hipSuccess) {
+      release(h);
+      delete h;
+      return fail("ragged class tables", e);
+   }
    if (scr) {
       if ((e = hipMalloc((void **) &h->scratch, scr)) != hipSuccess) {
          release(h);
@@ -261,7 +375,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       delete h;
       return fail("hipStreamCreate", e);
    }
-   const int lanes = (int) std::min<size_t>(kCaptureLanes, h->groups.size());
+   const int lanes = kCaptureLanes;
    for (int i = 0; i < lanes; ++i) {
       hipStream_t s;
       if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) {
@@ -271,8 +385,8 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       }
       h->lanes.push_back(s);
    }
-   // fork + joins for up to two directions
-   const size_t nev = 2 * (1 + h->lanes.size());
+   // two fork/join phases for each of up to two directions
+   const size_t nev = 2 * 2 * (1 + h->lanes.size());
    for (size_t i = 0; i < nev; ++i) {
       hipEvent_t ev;
       if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
@@ -313,11 +427,9 @@ FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB
       // capture; the warm-up runs after the caller's pending work on pX / pY
       if (handle->exec == nullptr) {
          if ((e = order_after_caller(handle)) != hipSuccess) return fail("ragged stream ordering", e);
-         for (const auto &g : handle->groups) {
-            if ((e = launch_group(g, pX, pY, C2C_FORWARD, handle->scratch, g.maxGrid, handle->capture)) !=
-                hipSuccess)
-               return fail("ragged warm-up launch", e);
-         }
+         size_t ev = 0;
+         if ((e = enqueue_direction(handle, pX, pY, C2C_FORWARD, handle->capture, ev)) != hipSuccess)
+            return fail("ragged warm-up launch", e);
          if ((e = hipStreamSynchronize(handle->capture)) != hipSuccess) return fail("ragged warm-up", e);
       }
       if ((e = capture_graph(handle, pX, pY, pZ, direction)) != hipSuccess) return fail("ragged graph capture", e);
@@ -342,27 +454,30 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    if ((e = hipEventCreate(&t1)) != hipSuccess) return fail("hipEventCreate", e);
    hipStream_t s = handle->capture;
    if ((e = order_after_caller(handle)) != hipSuccess) return fail("ragged stream ordering", e);
-   for (auto &g : handle->groups) {
-      if (g.large) continue;   // the large path has no grid parameter
+   // the block-mode tasks read the outer passes' results from the scratch
+   for (const auto &g : handle->groups)
+      if (g.large && e == hipSuccess) e = launch_outer(*handle, g, pX, C2C_FORWARD, s);
+   for (auto &c : handle->classes) {
+      if (e != hipSuccess) break;
       // candidates: resident slots (0), then caps at multiples of the CU count
       const uint32_t cand[] = {0u, 4u * (uint32_t) cus, 2u * (uint32_t) cus, (uint32_t) cus, (uint32_t) cus / 2u};
       float          best   = 1e30f;
       uint32_t       bestG  = 0;
-      for (uint32_t c : cand) {
-         if ((e = launch_group(g, pX, pY, C2C_FORWARD, handle->scratch, c, s)) != hipSuccess) break;  // warm
+      for (uint32_t cap : cand) {
+         if ((e = launch_class(*handle, c, pX, pY, C2C_FORWARD, cap, s)) != hipSuccess) break;   // warm
          float ms = 0.0f;
          if ((e = hipEventRecord(t0, s)) != hipSuccess) break;
-         for (int r = 0; r < 3 && e == hipSuccess; ++r) e = launch_group(g, pX, pY, C2C_FORWARD, handle->scratch, c, s);
+         for (int r = 0; r < 3 && e == hipSuccess; ++r) e = launch_class(*handle, c, pX, pY, C2C_FORWARD, cap, s);
          if (e != hipSuccess) break;
          if ((e = hipEventRecord(t1, s)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess ||
              (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess)
             break;
-         if (ms < best) best = ms, bestG = c;
+         if (ms < best) best = ms, bestG = cap;
       }
       if (e != hipSuccess) break;
-      g.maxGrid = bestG;
+      c.maxGrid = bestG;
       std::lock_guard<std::mutex> tl(g_tuneMu);
-      g_tune[TuneKey{g.n, bucket_of(g.count)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
+      g_tune[TuneKey{kClassKeyBase + (uint32_t) c.cls, bucket_of(c.numItems)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
    }
    (void) hipEventDestroy(t0);
    (void) hipEventDestroy(t1);

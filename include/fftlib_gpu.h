@@ -78,11 +78,12 @@ int32_t       FFTLIB_GPU_getC2CVariant(void);
  * handle's buffers.
  *
  * FFTLIB_GPU_raggedTune times the candidate persistent-grid sizes of every
- * group (one length) with the caller's buffers (pY is overwritten) and keeps
- * the fastest in a process-wide cache that later plans of the same lengths
- * and group sizes use; FFTLIB_GPU_tuneSave / FFTLIB_GPU_tuneLoad write and
- * read that cache as text ("fftlib-amd-tune 1", then one
- * "length group-size-bucket grid-cap microseconds" line per entry). */
+ * multi-length kernel the plan launches (one per octave of lengths) with the
+ * caller's buffers (pY is overwritten) and keeps the fastest in a
+ * process-wide cache that later plans with the same kernels and work sizes
+ * use; FFTLIB_GPU_tuneSave / FFTLIB_GPU_tuneLoad write and read that cache as
+ * text ("fftlib-amd-tune 1", then one "key item-count-bucket grid-cap
+ * microseconds" line per entry, key = 100000 + octave index). */
 typedef struct FFTLIB_GPU_ragged_s *FFTLIB_GPU_raggedHandle;
 FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uint32_t *lengths, uint32_t count);
 FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB_F32 *pX, FFTLIB_F32 *pY,
