@@ -34,8 +34,15 @@
 // ..._cn.c:35-87.
 #include <hip/hip_runtime.h>
 
+#include <math.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "fft_fixed_impl.h"
 #include "fft_fixed_class.h"
+#include "fftlib_gpu.h"
 
 namespace fftlib {
 using namespace fixedk;
@@ -146,10 +153,8 @@ hipError_t dispatch_c2c_any(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hi
    const int cls = fixed_class_of(n);
    if (cls < 0) return hipErrorInvalidValue;
    ClassArgs c{};
-   c.x = a.x, c.y = a.y, c.scratch = a.x, c.mode = a.mode;
-   c.single = single_task(n, a.numChannels, a.tw, a.inOff, a.outOff);
-   c.single.nBlocks = a.nBlocks, c.single.fullN = a.fullN, c.single.fullPow2 = a.fullPow2;
-   c.single.fullLog2 = a.fullLog2, c.single.fullDigits = a.fullDigits, c.single.fullDigitCode = a.fullDigitCode;
+   c.x = a.x, c.y = a.y, c.scratch = nullptr, c.mode = a.mode;
+   c.single   = single_task(n, a.numChannels, a.tw, a.inOff, a.outOff);
    c.numItems = c.single.numGroups;
    return fixed_class_launch(cls, c, maxGrid, stream);
 }
@@ -179,34 +184,6 @@ hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t 
    return dispatch_c2c_any(n, a, 0, stream);
 }
 
-hipError_t launch_c2c_fixed_ragged(const float *x, float *y, const float *tw, uint32_t n, uint32_t count, int mode,
-                                   const uint64_t *chanOff, uint32_t maxGrid, hipStream_t stream)
-{
-   if (count == 0) return hipSuccess;
-   FixedArgs a{};
-   a.x = (const float2 *) x, a.y = (float2 *) y, a.tw = (const float2 *) tw, a.sf = nullptr;
-   a.numChannels = count, a.mode = mode, a.aux = 0, a.inStride = n, a.outStride = n;
-   a.inOff = chanOff, a.outOff = chanOff;
-   return dispatch_c2c_any(n, a, maxGrid, stream);
-}
-
-hipError_t launch_c2c_fixed_blocks(const float *scratch, float *y, const float *twTail, uint32_t m2,
-                                   uint32_t numTransforms, const StageTable &full, const uint64_t *outOff, int mode,
-                                   hipStream_t stream)
-{
-   if (numTransforms == 0) return hipSuccess;
-   if (full.n % m2 != 0 || full.nDigits > 32) return hipErrorInvalidValue;
-   FixedArgs a{};
-   a.x = (const float2 *) scratch, a.y = (float2 *) y, a.tw = (const float2 *) twTail, a.sf = nullptr;
-   a.nBlocks     = full.n / m2;
-   a.numChannels = numTransforms * a.nBlocks;
-   a.mode = mode, a.aux = 0, a.inStride = m2, a.outStride = 0;
-   a.inOff = nullptr, a.outOff = outOff;
-   a.fullN = full.n, a.fullPow2 = full.pow2, a.fullLog2 = full.log2n, a.fullDigits = full.nDigits;
-   a.fullDigitCode = pack_digits(full.digit, full.nDigits);
-   return dispatch_c2c_any(m2, a, 0, stream);
-}
-
 hipError_t launch_r2c_fixed(const float *x, float *y, const float *tw, const float *sf, uint32_t nc,
                             uint32_t numChannels, uint32_t outmode, hipStream_t stream)
 {
@@ -227,6 +204,111 @@ hipError_t launch_c2r_fixed(const float *x, float *y, const float *tw, const flo
    a.numChannels = numChannels, a.mode = postdiv ? C2C_INVERSE_POSTDIV : C2C_INVERSE_PRESCALE, a.aux = inMode;
    a.inStride = inMode ? 2u * nc : nc + 1u, a.outStride = nc;
    return dispatch_fixed_real<KIND_C2R>(nc, a, stream);
+}
+
+// ---- two-pass transforms longer than 16384 points -------------------------------
+namespace {
+
+constexpr uint32_t kTwoPassMaxN = 1u << 22;   // column twiddle table: 8 B per point
+
+struct TwoPassKey {
+   int      dev;
+   uint32_t n;
+   bool     operator<(const TwoPassKey &o) const { return dev != o.dev ? dev < o.dev : n < o.n; }
+};
+std::mutex                          g_twoPassMu;
+std::map<TwoPassKey, TwoPassPlan>   g_twoPass;   // device tables live until the process ends
+
+}  // namespace
+
+bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2)
+{
+   if (n <= 16384u || n > kTwoPassMaxN || !smooth235(n)) return false;
+   uint32_t best = 0;
+   for (uint32_t d = 16; d * d <= n; ++d) {
+      if (n % d != 0u || n / d > 16384u) continue;
+      best = d;   // largest divisor <= sqrt(n): the most balanced split
+   }
+   if (best == 0) return false;
+   *m2 = best;
+   *n1 = n / best;
+   return true;
+}
+
+hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan)
+{
+   uint32_t n1 = 0, m2 = 0;
+   if (!two_pass_split(n, &n1, &m2)) return hipErrorInvalidValue;
+   int        dev = 0;
+   hipError_t e   = hipGetDevice(&dev);
+   if (e != hipSuccess) return e;
+   std::lock_guard<std::mutex> lock(g_twoPassMu);
+   auto it = g_twoPass.find(TwoPassKey{dev, n});
+   if (it != g_twoPass.end()) {
+      *plan = it->second;
+      return hipSuccess;
+   }
+   // [table of n1 | table of m2 | W_n^e, e < n], complex fp32 from double
+   std::vector<float> host(2u * ((size_t) n1 + m2 + n));
+   FFTLIB_tw_gen_c2c(host.data(), (int32_t) n1);
+   FFTLIB_tw_gen_c2c(host.data() + 2u * n1, (int32_t) m2);
+   float *col = host.data() + 2u * ((size_t) n1 + m2);
+   for (uint32_t k = 0; k < n; ++k) {
+      const double a = 2.0 * M_PI * (double) k / (double) n;
+      col[2u * k]      = (float) cos(a);
+      col[2u * k + 1u] = (float) -sin(a);
+   }
+   float *d = nullptr;
+   if ((e = hipMalloc((void **) &d, host.size() * sizeof(float))) != hipSuccess) return e;
+   if ((e = hipMemcpy(d, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
+      (void) hipFree(d);
+      return e;
+   }
+   TwoPassPlan p;
+   p.n = n, p.n1 = n1, p.m2 = m2;
+   p.tw1 = d, p.tw2 = d + 2u * n1, p.colTw = d + 2u * ((size_t) n1 + m2);
+   g_twoPass[TwoPassKey{dev, n}] = p;
+   *plan = p;
+   return hipSuccess;
+}
+
+ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const uint64_t *off, uint64_t scratchOff)
+{
+   ClassTask k{};
+   const bool cols = layout == (int) LAYOUT_COLUMNS;
+   k.n           = cols ? p.n1 : p.m2;
+   k.numChannels = count * (cols ? p.m2 : p.n1);
+   k.numGroups   = (k.numChannels + fixed_fpb(k.n) - 1u) / fixed_fpb(k.n);
+   k.layout      = (uint32_t) layout;
+   k.tw          = (const float2 *) (cols ? p.tw1 : p.tw2);
+   k.inOff       = cols ? off : nullptr;
+   k.outOff      = cols ? nullptr : off;
+   k.scratchOff  = scratchOff;
+   k.span        = cols ? p.m2 : p.n1;
+   k.fullN       = p.n;
+   k.colTw       = (const float2 *) p.colTw;
+   return k;
+}
+
+uint32_t class_group_size(uint32_t n) { return fixed_fpb(n); }
+
+hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, float *scratch,
+                               hipStream_t stream)
+{
+   if (numChannels == 0) return hipSuccess;
+   TwoPassPlan p;
+   hipError_t  e = two_pass_plan(n, &p);
+   if (e != hipSuccess) return e;
+   if ((uint64_t) numChannels * p.m2 >= (1ull << 32) || (uint64_t) numChannels * p.n1 >= (1ull << 32))
+      return hipErrorInvalidValue;
+   ClassArgs c{};
+   c.x = (const float2 *) x, c.y = (float2 *) y, c.scratch = (float2 *) scratch, c.mode = mode;
+   c.single   = two_pass_task(p, LAYOUT_COLUMNS, numChannels, nullptr, 0);
+   c.numItems = c.single.numGroups;
+   if ((e = fixed_class_launch(fixed_class_of(p.n1), c, 0, stream)) != hipSuccess) return e;
+   c.single   = two_pass_task(p, LAYOUT_ROWS, numChannels, nullptr, 0);
+   c.numItems = c.single.numGroups;
+   return fixed_class_launch(fixed_class_of(p.m2), c, 0, stream);
 }
 
 }  // namespace fftlib

@@ -11,6 +11,7 @@
 namespace fftlib {
 namespace fixedk {
 struct ClassArgs;
+struct ClassTask;
 }
 
 constexpr int kFixedClasses = 11;
@@ -27,5 +28,37 @@ hipError_t fixed_class_launch_tu0(int cls, const fixedk::ClassArgs &c, uint32_t 
 hipError_t fixed_class_launch_tu1(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
 hipError_t fixed_class_launch_tu2(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
 hipError_t fixed_class_launch_tu3(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
+
+// Workgroup group size (transforms per workgroup) of length n's class kernel.
+uint32_t class_group_size(uint32_t n);
+
+// Two-pass transforms longer than 16384 points: n = n1 * m2 with both factors
+// class-kernel lengths. The column pass (class of n1) runs the n1-point DFTs
+// of the m2 columns and applies W_n^(p*k1); the row pass (class of m2) runs
+// the m2-point DFTs of the n1 rows and writes the outputs in natural order.
+struct TwoPassPlan {
+   uint32_t     n, n1, m2;
+   const float *tw1, *tw2;  // twiddle tables of n1 and m2 (device)
+   const float *colTw;      // W_n^e, e < n (device)
+};
+
+// Most balanced split of n (false when n <= 16384, n > 2^22 or not 2/3/5-smooth).
+bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2);
+
+// Plan of n on the current device; its tables are built on first use and kept
+// until the process ends.
+hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan);
+
+// Class-kernel task of one pass over `count` transforms: layout
+// LAYOUT_COLUMNS (off = input offsets of the transforms, or null) or
+// LAYOUT_ROWS (off = output offsets); scratchOff = complex offset of the
+// transforms' n*count-point region of the scratch buffer.
+fixedk::ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const uint64_t *off,
+                                uint64_t scratchOff);
+
+// numChannels contiguous transforms of n points, x -> y, through `scratch`
+// (n * numChannels complex values); mode as in launch_c2c.
+hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, float *scratch,
+                               hipStream_t stream);
 
 }  // namespace fftlib

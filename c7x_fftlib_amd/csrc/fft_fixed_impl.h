@@ -212,54 +212,24 @@ struct FixedArgs {
    uint32_t      inStride;    // complex points per input channel
    uint32_t      outStride;   // complex points per output channel
    const uint64_t *inOff;     // ragged batches: complex offset of each channel in x (else null)
-   const uint64_t *outOff;    // ... and in y (of each full transform in block mode)
-   // Block mode (the LDS stage of transforms longer than one workgroup):
-   // channel g is sub-block b = g % nBlocks of transform f = g / nBlocks,
-   // read contiguously from x; its output kk goes to element
-   // c_b + nBlocks*kk of transform f in y, where c_b is the frequency whose
-   // in-place DIF position of the full transform is b*N (N = this kernel's
-   // length). The full transform's digit sequence gives c_b.
-   uint32_t nBlocks;          // 0 = not block mode
-   uint32_t fullN;
-   uint32_t fullPow2, fullLog2, fullDigits;
-   uint64_t fullDigitCode;    // digit i (lsd first) in bits 2i..2i+1: 0 -> 2, 1 -> 3, 2 -> 5
+   const uint64_t *outOff;    // ... and in y (of each full transform in the two-pass layouts)
+   // Two-pass layouts of a transform longer than one workgroup, fullN =
+   // N1 * M2, input n = p + M2*i, output k = k1 + N1*k2:
+   //   LAYOUT_COLUMNS (this kernel's N = N1, span = M2): channel f*M2 + p is
+   //     column p of transform f, points x[in(f) + p + M2*i]; its outputs
+   //     Z[k1] = W_fullN^(p*k1) * Y[k1] go to y[f*fullN + k1*M2 + p];
+   //   LAYOUT_ROWS (N = M2, span = N1): channel f*N1 + k1 is row k1 of the
+   //     column results (contiguous, x[(f*N1 + k1)*M2 + p]); its outputs go
+   //     to y[out(f) + k1 + N1*k2].
+   // A workgroup's FPB channels are adjacent columns (rows), so every access
+   // to HBM is a run of FPB consecutive points.
+   uint32_t      layout;
+   uint32_t      span;
+   uint32_t      fullN;
+   const float2 *colTw;       // LAYOUT_COLUMNS: W_fullN^e for e < fullN
 };
 
-// Packs a digit sequence of radices 2/3/5 (at most 32) into FixedArgs's code.
-__host__ __device__ inline uint64_t pack_digits(const uint8_t *d, uint32_t n)
-{
-   uint64_t code = 0;
-   for (uint32_t i = 0; i < n; ++i) code |= (uint64_t) (d[i] == 2 ? 0u : d[i] == 3 ? 1u : 2u) << (2u * i);
-   return code;
-}
-
-// Frequency of the full transform whose in-place DIF position is P.
-__device__ __forceinline__ uint32_t full_dif_inverse(const FixedArgs &a, uint32_t P)
-{
-   if (a.fullPow2) return a.fullLog2 ? (__brev(P) >> (32u - a.fullLog2)) : 0u;
-   uint32_t k = 0;
-   for (int i = (int) a.fullDigits - 1; i >= 0; --i) {
-      const uint32_t c  = (uint32_t) (a.fullDigitCode >> (2u * (uint32_t) i)) & 3u;
-      const uint32_t r  = c == 0u ? 2u : c == 1u ? 3u : 5u;
-      const uint32_t qd = P / r;
-      k                 = k * r + (P - qd * r);
-      P                 = qd;
-   }
-   return k;
-}
-
-// First group a persistent workgroup takes. In block mode the sub-blocks of
-// one transform (consecutive groups) go to workgroups of the same XCD
-// (dispatch places workgroup w on XCD w % 8) in the same round, so the
-// interleaved outputs of one transform meet in that XCD's L2.
-__device__ __forceinline__ uint32_t first_group(const FixedArgs &a)
-{
-   const uint32_t w = blockIdx.x, grid = gridDim.x;
-   if (a.nBlocks == 0u || grid < 8u || (grid & 7u) != 0u) return w;
-   return (w & 7u) * (grid >> 3) + (w >> 3);
-}
-
-
+enum : uint32_t { LAYOUT_PLAIN = 0u, LAYOUT_COLUMNS = 1u, LAYOUT_ROWS = 2u };
 
 // First complex element of channel ch in the input and in the output.
 __device__ __forceinline__ size_t in_base(const FixedArgs &a, uint32_t ch)
@@ -268,11 +238,6 @@ __device__ __forceinline__ size_t in_base(const FixedArgs &a, uint32_t ch)
 }
 __device__ __forceinline__ size_t out_base(const FixedArgs &a, uint32_t ch)
 {
-   if (a.nBlocks) {
-      const uint32_t f = ch / a.nBlocks, b = ch - f * a.nBlocks;
-      const size_t   t = a.outOff ? (size_t) a.outOff[f] : (size_t) f * a.fullN;
-      return t + full_dif_inverse(a, b * (uint32_t) (a.fullN / a.nBlocks));
-   }
    return a.outOff ? (size_t) a.outOff[ch] : (size_t) ch * a.outStride;
 }
 
@@ -383,11 +348,11 @@ __device__ __forceinline__ float2 out_postdiv(float2 v, float n) { return make_f
 // its transform (and, for R2C, the mirrored N - k). They are the same for
 // every group of a persistent workgroup, so they are computed once when they
 // fit in a few registers (at most 16 outputs per work item).
-template <int N, int KIND>
+template <int N, int KIND, bool CACHE = true>
 struct StoreMap {
    static constexpr FixedPlan P      = Fixed<N>::P;
    static constexpr int       iters  = (N + P.T - 1) / P.T;
-   static constexpr bool      cached = iters <= 16;
+   static constexpr bool      cached = CACHE && iters <= 16;
    static constexpr int       slots  = cached ? iters : 1;
    uint32_t                   off[slots];
    uint32_t                   mir[KIND == KIND_R2C ? slots : 1];
@@ -424,18 +389,17 @@ struct StoreMap {
 // order; R2C applies the split Y[i] = Z[i] A[i] + (Z[N-i] mirrored) B[i] for
 // i < N, the Nyquist bin Y[N] = Z0re - Z0im, and for outmode 1 the conjugate
 // upper half Y[2N-i] = conj(Y[i]).
-template <int N, int KIND>
+template <int N, int KIND, bool C>
 __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint32_t t, const FixedArgs &a,
-                                            const StoreMap<N, KIND> &map)
+                                            const StoreMap<N, KIND, C> &map)
 {
    constexpr FixedPlan P     = Fixed<N>::P;
    constexpr int       T     = P.T;
-   constexpr int       iters = StoreMap<N, KIND>::iters;
+   constexpr int       iters = StoreMap<N, KIND, C>::iters;
    constexpr int       MAXC  = KIND == KIND_R2C ? 8 : 16;
    constexpr int       CH    = iters < MAXC ? iters : MAXC;
    constexpr bool      EXACT = N % T == 0 && iters % CH == 0;
    float2             *out   = a.y + out_base(a, ch);
-   const uint32_t      es    = a.nBlocks ? a.nBlocks : 1u;   // output element stride
    const float         sy    = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
    const bool          pdiv  = a.mode == C2C_INVERSE_POSTDIV;
 #pragma unroll 1
@@ -468,7 +432,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k * es] = out_postdiv(v[c], (float) (a.nBlocks ? a.fullN : (uint32_t) N));
+            out[k] = out_postdiv(v[c], (float) N);
          }
       }
       else {
@@ -476,7 +440,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k * es] = make_float2(v[c].x, sy * v[c].y);
+            out[k] = make_float2(v[c].x, sy * v[c].y);
          }
       }
    }
@@ -532,7 +496,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
    extern __shared__ float2 smem[];
    const uint32_t fl   = threadIdx.x / T;
    const uint32_t t    = threadIdx.x % T;
-   const InScale  sc   = a.nBlocks ? InScale{1.0f, 1.0f} : in_scale(a.mode, 1.0f / (float) N);
+   const InScale  sc   = in_scale(a.mode, 1.0f / (float) N);
    float2        *data = smem + fl * STRIDE;
    const float2  *twp  = fixed_tw_lds<N>() ? (const float2 *) (smem + FPB * STRIDE) : a.tw;
    StoreMap<N, KIND> map;
@@ -541,7 +505,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
    if constexpr (fixed_prefetch<N, KIND>()) {
       using Q = Pass0<N>;
       float2   v[Q::iters][Q::R];
-      uint32_t g = first_group(a);
+      uint32_t g = blockIdx.x;
       if (g < a.numGroups) pass0_load<N, KIND>(v, g * FPB, fl, min((uint32_t) FPB, a.numChannels - g * FPB), t, a);
       if constexpr (fixed_tw_lds<N>()) {
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
@@ -568,7 +532,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       }
       constexpr int iters = N / T;
       constexpr int CH    = largest_divisor_upto(iters, 8);
-      for (uint32_t g = first_group(a); g < a.numGroups; g += gridDim.x) {
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
          if (fl < nf) {
@@ -595,7 +559,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
          __syncthreads();
       }
-      for (uint32_t g = first_group(a); g < a.numGroups; g += gridDim.x) {
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
 #pragma unroll 1
@@ -614,7 +578,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       if constexpr (fixed_tw_lds<N>()) {
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
       }
-      for (uint32_t g = first_group(a); g < a.numGroups; g += gridDim.x) {
+      for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
          for (uint32_t i = threadIdx.x; i < nf * N; i += kFixedThreads) {
@@ -642,21 +606,21 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
 // one task passed by value.
 struct ClassTask {
    uint32_t        n;            // one of the class's lengths
-   uint32_t        numChannels;  // channels (transforms x sub-blocks in block mode)
+   uint32_t        numChannels;  // channels (columns / rows in the two-pass layouts)
    uint32_t        numGroups;
-   uint32_t        fromScratch;  // block mode: input from the scratch buffer, not x
-   const float2   *tw;
-   const uint64_t *inOff;
-   const uint64_t *outOff;
-   uint64_t        scratchOff;   // complex offset of this task's scratch region
-   uint32_t        nBlocks, fullN, fullPow2, fullLog2, fullDigits;
-   uint64_t        fullDigitCode;
+   uint32_t        layout;       // LAYOUT_PLAIN / LAYOUT_COLUMNS / LAYOUT_ROWS
+   const float2   *tw;           // twiddle table of length n
+   const uint64_t *inOff;        // plain and columns: complex offset of each transform in x
+   const uint64_t *outOff;       // plain and rows: ... in y
+   uint64_t        scratchOff;   // two-pass: complex offset of the task's scratch region
+   uint32_t        span, fullN;  // two-pass: see FixedArgs
+   const float2   *colTw;
 };
 
 struct ClassArgs {
    const float2    *x;
    float2          *y;
-   const float2    *scratch;
+   float2          *scratch;    // two-pass intermediate (columns write, rows read)
    int              mode;       // C2CMode
    uint32_t         numItems;
    const ClassTask *tasks;      // null: one task, `single`, with item = group index
@@ -666,16 +630,130 @@ struct ClassArgs {
 
 constexpr uint32_t kClassGroupBits = 20;
 
+// Per-transform LDS stride of the class kernels: odd, so that the FPB
+// channels of a column or row group (lanes j = 0..FPB-1 at the same point)
+// fall in distinct LDS banks.
+template <int N>
+constexpr uint32_t class_stride()
+{
+   return fixed_stride<N>() | 1u;
+}
+
+// Column group: the FPB adjacent columns of the group are loaded row by row
+// (lane j of a row takes column j), transformed in LDS, multiplied by
+// W_fullN^(p*k1) and stored row by row.
+template <int N>
+__device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, float2 *smem)
+{
+   constexpr FixedPlan P      = Fixed<N>::P;
+   constexpr int       T      = P.T;
+   constexpr int       FPB    = P.FPB;
+   constexpr uint32_t  STRIDE = class_stride<N>();
+   constexpr int       IT     = (N + T - 1) / T;   // points per lane
+   constexpr int       CH     = IT < 16 ? IT : 16;
+   const uint32_t      j      = threadIdx.x % FPB;
+   const uint32_t      i0     = threadIdx.x / FPB;
+   const uint32_t      ch     = g * FPB + j;
+   const bool          live   = ch < a.numChannels;
+   const uint32_t      chc    = live ? ch : g * FPB;   // lanes past the batch re-read the first column
+   const uint32_t      f      = chc / a.span;
+   const uint32_t      p      = chc - f * a.span;
+   const float2       *in     = a.x + (a.inOff ? (size_t) a.inOff[f] : (size_t) f * a.fullN) + p;
+   const InScale       sc     = in_scale(a.mode, 1.0f / (float) a.fullN);
+   float2             *col    = smem + j * STRIDE;
+#pragma unroll 1
+   for (int c0 = 0; c0 < IT; c0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+         const uint32_t i = min(i0 + (uint32_t) ((c0 + c) * T), (uint32_t) N - 1u);
+         v[c]             = in[(size_t) a.span * i];
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+         const uint32_t i = i0 + (uint32_t) ((c0 + c) * T);
+         if ((N % T == 0 && IT % CH == 0) || (c0 + c < IT && i < (uint32_t) N))
+            col[fslot(i)] = make_float2(sc.sx * v[c].x, sc.sy * v[c].y);
+      }
+   }
+   __syncthreads();
+   fixed_passes_from<N, 0>(smem + (threadIdx.x / T) * STRIDE, threadIdx.x % T, a.tw);
+   if (live) {
+      float2 *out = a.y + (size_t) f * a.fullN + p;
+#pragma unroll 1
+      for (int c0 = 0; c0 < IT; c0 += CH) {
+         float2 v[CH], w[CH];
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t k = min(i0 + (uint32_t) ((c0 + c) * T), (uint32_t) N - 1u);
+            v[c]             = col[fslot(fixed_position<N>(k))];
+            w[c]             = a.colTw[p * k];
+         }
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t k = i0 + (uint32_t) ((c0 + c) * T);
+            if ((N % T == 0 && IT % CH == 0) || (c0 + c < IT && k < (uint32_t) N))
+               out[(size_t) a.span * k] = make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
+                                                      v[c].x * w[c].y + v[c].y * w[c].x);
+         }
+      }
+   }
+   __syncthreads();
+}
+
+// Row store: lane j of the group writes row first + j; the FPB rows of a
+// group are adjacent frequencies k1, so each output k2 is a run of FPB
+// consecutive points. `map` holds the LDS slots of outputs i0 + T*it,
+// i0 = threadIdx.x / FPB.
+template <int N, bool C>
+__device__ __forceinline__ void rows_store(const float2 *smem, uint32_t first, const FixedArgs &a,
+                                           const StoreMap<N, KIND_C2C, C> &map)
+{
+   constexpr FixedPlan P      = Fixed<N>::P;
+   constexpr int       T      = P.T;
+   constexpr int       FPB    = P.FPB;
+   constexpr uint32_t  STRIDE = class_stride<N>();
+   constexpr int       IT     = StoreMap<N, KIND_C2C, C>::iters;
+   constexpr int       CH     = IT < 16 ? IT : 16;
+   const uint32_t      j      = threadIdx.x % FPB;
+   const uint32_t      i0     = threadIdx.x / FPB;
+   const uint32_t      ch     = first + j;
+   if (ch >= a.numChannels) return;
+   const uint32_t f   = ch / a.span;
+   const uint32_t r   = ch - f * a.span;
+   float2        *out = a.y + (a.outOff ? (size_t) a.outOff[f] : (size_t) f * a.fullN) + r;
+   const float2  *src = smem + j * STRIDE;
+   const float    sy  = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
+   const bool     pdiv = a.mode == C2C_INVERSE_POSTDIV;
+   const float    nf  = (float) a.fullN;
+#pragma unroll 1
+   for (int c0 = 0; c0 < IT; c0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) v[c] = src[map.at(c0 + c, i0)];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+         const uint32_t k = i0 + (uint32_t) ((c0 + c) * T);
+         if (!(N % T == 0 && IT % CH == 0) && (c0 + c >= IT || k >= (uint32_t) N)) break;
+         out[(size_t) a.span * k] = pdiv ? out_postdiv(v[c], nf) : make_float2(v[c].x, sy * v[c].y);
+      }
+   }
+}
+
 template <int N, int KIND>
 __device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, float2 *smem)
 {
    constexpr FixedPlan P      = Fixed<N>::P;
    constexpr int       T      = P.T;
    constexpr int       FPB    = P.FPB;
-   constexpr uint32_t  STRIDE = fixed_stride<N>();
+   constexpr uint32_t  STRIDE = class_stride<N>();
+   if (a.layout == LAYOUT_COLUMNS) {
+      column_group<N>(a, g, smem);
+      return;
+   }
    const uint32_t      fl     = threadIdx.x / T;
    const uint32_t      t      = threadIdx.x % T;
-   const InScale       sc     = a.nBlocks ? InScale{1.0f, 1.0f} : in_scale(a.mode, 1.0f / (float) N);
+   const InScale       sc     = a.layout == LAYOUT_ROWS ? InScale{1.0f, 1.0f} : in_scale(a.mode, 1.0f / (float) N);
    float2             *data   = smem + fl * STRIDE;
    const uint32_t      first  = g * FPB;
    const uint32_t      nf     = min((uint32_t) FPB, a.numChannels - first);
@@ -701,38 +779,46 @@ __device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, floa
       fixed_passes_from<N, 0>(data, t, a.tw);
    }
    StoreMap<N, KIND> map;
-   map.init(t);
-   if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+   if (a.layout == LAYOUT_ROWS) {
+      map.init(threadIdx.x / FPB);
+      rows_store<N>(smem, first, a, map);
+   }
+   else {
+      map.init(t);
+      if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+   }
    __syncthreads();
 }
 
 __device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const ClassTask &k)
 {
    FixedArgs a{};
-   a.x           = k.fromScratch ? c.scratch + k.scratchOff : c.x;
-   a.y           = c.y;
+   a.x           = k.layout == LAYOUT_ROWS ? c.scratch + k.scratchOff : c.x;
+   a.y           = k.layout == LAYOUT_COLUMNS ? c.scratch + k.scratchOff : c.y;
    a.tw          = k.tw;
    a.numChannels = k.numChannels;
    a.numGroups   = k.numGroups;
    a.mode        = c.mode;
    a.inStride    = k.n;
    a.outStride   = k.n;
-   a.inOff       = k.inOff;
-   a.outOff      = k.outOff;
-   a.nBlocks     = k.nBlocks;
+   a.inOff       = k.layout == LAYOUT_ROWS ? nullptr : k.inOff;
+   a.outOff      = k.layout == LAYOUT_COLUMNS ? nullptr : k.outOff;
+   a.layout      = k.layout;
+   a.span        = k.span;
    a.fullN       = k.fullN;
-   a.fullPow2    = k.fullPow2;
-   a.fullLog2    = k.fullLog2;
-   a.fullDigits    = k.fullDigits;
-   a.fullDigitCode = k.fullDigitCode;
+   a.colTw       = k.colTw;
    return a;
 }
 
-// One group of task *tk, compiled per length as a separate function so that
+// Group g of task *tk, compiled per length as a separate function so that
 // each length gets its own register allocation (all lengths inlined into one
-// kernel body need several times the registers of the largest one). The
-// task and the kernel arguments are passed by address (kernel-argument or
-// global memory), the data through the kernel's dynamic LDS.
+// kernel body need several times the registers of the largest one). The task
+// and the kernel arguments are passed by address (kernel-argument or global
+// memory), the data through the kernel's dynamic LDS. Twiddles are read from
+// global memory (L2). Measured alternatives that were slower on MI355X:
+// twiddles staged in LDS (per group or per task), a loop over a task's groups
+// inside one call (its hoisted per-thread indexing spills at the register
+// budget), and prefetching the next group's points into registers.
 template <int N>
 __device__ __attribute__((noinline)) void class_group(const ClassTask *tk, const ClassArgs *c, uint32_t g)
 {
@@ -741,8 +827,34 @@ __device__ __attribute__((noinline)) void class_group(const ClassTask *tk, const
    fixed_group<N, KIND_C2C>(a, g, smem_class);
 }
 
+template <int N>
+constexpr size_t class_length_lds()
+{
+   return (size_t) Fixed<N>::P.FPB * class_stride<N>() * sizeof(float2);
+}
+
 template <int... Ls>
-__global__ void __launch_bounds__(kFixedThreads) fixed_class_kernel(ClassArgs c)
+constexpr size_t class_lds_bytes()
+{
+   size_t m = 0;
+   ((m = m > class_length_lds<Ls>() ? m : class_length_lds<Ls>()), ...);
+   return m;
+}
+
+// Waves per SIMD the class's LDS allows (one wave of each resident
+// workgroup per SIMD, 160 KB of LDS per CU, at most 4): the register budget
+// of the kernel and of every per-length function it calls follows from it
+// (512 / waves VGPRs).
+template <int... Ls>
+constexpr int class_waves()
+{
+   const size_t w = (size_t) 163840 / class_lds_bytes<Ls...>();
+   return w < 1 ? 1 : w > 4 ? 4 : (int) w;
+}
+
+template <int... Ls>
+__global__ void __launch_bounds__(kFixedThreads) __attribute__((amdgpu_waves_per_eu(class_waves<Ls...>())))
+fixed_class_kernel(ClassArgs c)
 {
    for (uint32_t w = blockIdx.x; w < c.numItems; w += gridDim.x) {
       uint32_t         g  = w;
@@ -755,17 +867,6 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_class_kernel(ClassArgs c)
       const uint32_t n = tk->n;
       (void) ((n == (uint32_t) Ls ? (class_group<Ls>(tk, &c, g), true) : false) || ...);
    }
-}
-
-template <int... Ls>
-constexpr size_t class_lds_bytes()
-{
-   size_t m = 0;
-   ((m = m > (size_t) Fixed<Ls>::P.FPB * fixed_stride<Ls>() * sizeof(float2)
-            ? m
-            : (size_t) Fixed<Ls>::P.FPB * fixed_stride<Ls>() * sizeof(float2)),
-    ...);
-   return m;
 }
 
 template <int... Ls>

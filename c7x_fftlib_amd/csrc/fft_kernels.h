@@ -61,13 +61,12 @@ uint32_t max_lds_points();
 // LDS, i.e. launch_c2c / launch_r2c / launch_c2r can run it.
 bool fits_lds(const StageTable &st);
 
-// Complex FFTs longer than one workgroup's LDS (fft_large.hip): outer radix
-// passes over HBM into `scratch` (st.n * numChannels complex values), then
-// sub-blocks finished in LDS. Same modes as launch_c2c. chanOff (optional,
-// device array) gives each channel's complex offset in x and y (ragged
-// batches); scratch stays contiguous.
+// Complex FFTs longer than one workgroup's LDS on the generic kernels
+// (fft_large.hip): outer radix passes over HBM into `scratch`
+// (st.n * numChannels complex values), then sub-blocks finished in LDS. Same
+// modes as launch_c2c.
 hipError_t launch_c2c_large(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
-                            int mode, float *scratch, hipStream_t stream, const uint64_t *chanOff = nullptr);
+                            int mode, float *scratch, hipStream_t stream);
 
 // Lengths with a compile-time specialised complex kernel (fft_fixed.hip).
 bool fixed_c2c_supported(uint32_t n);
@@ -76,29 +75,6 @@ bool fixed_c2c_supported(uint32_t n);
 // fixed_c2c_supported(n)); same semantics as launch_c2c.
 hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t n, uint32_t numChannels, int mode,
                             hipStream_t stream);
-
-// Ragged form: `count` transforms of n points at complex offsets chanOff[i]
-// (a device array) of x and y. maxGrid > 0 caps the persistent grid.
-hipError_t launch_c2c_fixed_ragged(const float *x, float *y, const float *tw, uint32_t n, uint32_t count, int mode,
-                                   const uint64_t *chanOff, uint32_t maxGrid, hipStream_t stream);
-
-// First pass of the sub-blocks of the large path (0 when the length has no
-// pass whose sub-length fits the LDS stage), and the outer passes alone
-// (x -> scratch), for callers that run the LDS stage themselves.
-uint32_t   large_first_inner_pass(const StageTable &st);
-hipError_t launch_c2c_large_outer(const float *x, const float *tw, const StageTable &st, uint32_t numChannels, int mode,
-                                  float *scratch, hipStream_t stream, const uint64_t *chanOff);
-
-// LDS stage of a transform longer than one workgroup's LDS on the specialised
-// kernel of the sub-block length m2 (fixed_c2c_supported(m2)): numTransforms
-// transforms of full.n points whose first passes left contiguous sub-blocks of
-// m2 points in `scratch`; twTail = the full table from the first stage of the
-// sub-blocks. Outputs go to natural order in y (outOff: ragged offsets of the
-// full transforms, or null for contiguous ones); mode as in launch_c2c, the
-// input conj/scale having been applied by the first outer pass.
-hipError_t launch_c2c_fixed_blocks(const float *scratch, float *y, const float *twTail, uint32_t m2,
-                                   uint32_t numTransforms, const StageTable &full, const uint64_t *outOff, int mode,
-                                   hipStream_t stream);
 
 // Inner complex lengths nc (= N/2) with specialised real-transform kernels.
 bool fixed_real_supported(uint32_t nc);

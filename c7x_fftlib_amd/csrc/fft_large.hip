@@ -1,6 +1,7 @@
-// Complex FFTs longer than one workgroup's LDS (the reference's C2C kernels
-// accept any 2/3/5-smooth length; its largest published case is 16384
-// points, and ragged batches here go to 65536).
+// Complex FFTs longer than one workgroup's LDS on the generic (runtime stage
+// table) kernels. The specialised path runs these lengths in two passes on
+// the class kernels (fft_fixed.hip); this path serves the generic kernels and
+// lengths beyond the two-pass range.
 //
 // The first register passes run over HBM (outer_pass_kernel: each work item
 // loads its R = r1*r2 points at stride N/R or M/R, applies one or two DIF
@@ -46,8 +47,7 @@ __device__ __forceinline__ uint32_t dif_inverse(uint32_t P, const StageTable &st
 template <int R1, int R2, int MODE, bool FIRST>
 __global__ void __launch_bounds__(kThreadsL) outer_pass_kernel(const float2 *src, float2 *dst,
                                                                const float2 *__restrict__ tw, StageTable st,
-                                                               uint32_t s, uint32_t numChannels,
-                                                               const uint64_t *chanOff)
+                                                               uint32_t s, uint32_t numChannels)
 {
    constexpr int  R     = R1 * R2;
    const uint32_t N     = st.n;
@@ -66,11 +66,10 @@ __global__ void __launch_bounds__(kThreadsL) outer_pass_kernel(const float2 *src
    const float    scale = 1.0f / (float) N;
    for (uint32_t f = blockIdx.y; f < numChannels; f += gridDim.y) {
       const size_t base  = (size_t) f * N + loc;
-      const size_t sbase = (FIRST && chanOff) ? (size_t) chanOff[f] + loc : base;
       float2       v[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-         float2 a = src[sbase + (size_t) q2 * j];
+         float2 a = src[base + (size_t) q2 * j];
          if (FIRST && MODE != C2C_FORWARD) a.y = -a.y;
          if (FIRST && MODE == C2C_INVERSE_PRESCALE) {
             a.x = scale * a.x;
@@ -95,8 +94,7 @@ template <int MODE>
 __global__ void __launch_bounds__(kThreadsL) inner_block_kernel(const float2 *__restrict__ t,
                                                                 float2 *__restrict__ y,
                                                                 const float2 *__restrict__ tw, StageTable st,
-                                                                uint32_t p0, uint32_t M2, uint32_t numChannels,
-                                                                const uint64_t *chanOff)
+                                                                uint32_t p0, uint32_t M2, uint32_t numChannels)
 {
    extern __shared__ float2 smem[];
    const uint32_t N       = st.n;
@@ -124,13 +122,13 @@ __global__ void __launch_bounds__(kThreadsL) inner_block_kernel(const float2 *__
          v.x = v.x / (float) N;
          v.y = -(v.y / (float) N);
       }
-      y[(chanOff ? (size_t) chanOff[f] : (size_t) f * N) + k] = v;
+      y[(size_t) f * N + k] = v;
    }
 }
 
 template <int MODE, bool FIRST>
 hipError_t launch_outer(const float2 *src, float2 *dst, const float2 *tw, const StageTable &st, uint32_t pass,
-                        uint32_t numChannels, const uint64_t *chanOff, hipStream_t stream)
+                        uint32_t numChannels, hipStream_t stream)
 {
    const uint32_t s  = st.passFirst[pass];
    const uint32_t r1 = st.radix[s];
@@ -141,7 +139,7 @@ hipError_t launch_outer(const float2 *src, float2 *dst, const float2 *tw, const 
 #define FFTLIB_OUTER(A, B)                                                                                          \
    case A * 8 + B:                                                                                                   \
       hipLaunchKernelGGL((outer_pass_kernel<A, B, MODE, FIRST>), grid, block, 0, stream, src, dst, tw, st, s,       \
-                         numChannels, chanOff);                                                                      \
+                         numChannels);                                                                               \
       break;
    switch (r1 * 8 + r2) {
       FFTLIB_OUTER(5, 5)
@@ -164,19 +162,15 @@ hipError_t launch_outer(const float2 *src, float2 *dst, const float2 *tw, const 
 
 template <int MODE>
 hipError_t run_large(const float2 *x, float2 *y, const float2 *tw, const StageTable &st, uint32_t numChannels,
-                     float2 *scratch, uint32_t p0, const uint64_t *chanOff, hipStream_t stream)
+                     float2 *scratch, uint32_t p0, hipStream_t stream)
 {
    hipError_t e;
    for (uint32_t p = 0; p < p0; ++p) {
-      e = (p == 0) ? launch_outer<MODE, true>(x, scratch, tw, st, p, numChannels, chanOff, stream)
-                   : launch_outer<MODE, false>(scratch, scratch, tw, st, p, numChannels, nullptr, stream);
+      e = (p == 0) ? launch_outer<MODE, true>(x, scratch, tw, st, p, numChannels, stream)
+                   : launch_outer<MODE, false>(scratch, scratch, tw, st, p, numChannels, stream);
       if (e != hipSuccess) return e;
    }
    const uint32_t M2      = st.stageM[st.passFirst[p0]];
-   if (fixed_c2c_supported(M2))
-      return launch_c2c_fixed_blocks((const float *) scratch, (float *) y,
-                                     (const float *) (tw + st.twOff[st.passFirst[p0]]), M2, numChannels, st, chanOff,
-                                     MODE, stream);
    const size_t   lds     = (size_t) lds_points(1, M2) * sizeof(float2);
    const uint32_t total   = numChannels * (st.n / M2);
    const uint32_t per     = (total + kXcds - 1) / kXcds;
@@ -188,35 +182,8 @@ hipError_t run_large(const float2 *x, float2 *y, const float2 *tw, const StageTa
       ldsSet = true;
    }
    hipLaunchKernelGGL(inner_block_kernel<MODE>, dim3(per * kXcds), dim3(kThreadsL), lds, stream,
-                      (const float2 *) scratch, y, tw, st, p0, M2, numChannels, chanOff);
+                      (const float2 *) scratch, y, tw, st, p0, M2, numChannels);
    return hipGetLastError();
-}
-
-}  // namespace
-
-uint32_t large_first_inner_pass(const StageTable &st);
-
-hipError_t launch_c2c_large_outer(const float *x, const float *tw, const StageTable &st, uint32_t numChannels, int mode,
-                                  float *scratch, hipStream_t stream, const uint64_t *chanOff)
-{
-   if (numChannels == 0) return hipSuccess;
-   const uint32_t p0 = large_first_inner_pass(st);
-   if (p0 == 0) return hipErrorInvalidValue;
-   const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
-   float2       *t  = (float2 *) scratch;
-   hipError_t    e  = hipSuccess;
-   for (uint32_t p = 0; p < p0 && e == hipSuccess; ++p) {
-      if (mode == C2C_FORWARD)
-         e = p == 0 ? launch_outer<C2C_FORWARD, true>(xi, t, wi, st, p, numChannels, chanOff, stream)
-                    : launch_outer<C2C_FORWARD, false>(t, t, wi, st, p, numChannels, nullptr, stream);
-      else if (mode == C2C_INVERSE_PRESCALE)
-         e = p == 0 ? launch_outer<C2C_INVERSE_PRESCALE, true>(xi, t, wi, st, p, numChannels, chanOff, stream)
-                    : launch_outer<C2C_INVERSE_PRESCALE, false>(t, t, wi, st, p, numChannels, nullptr, stream);
-      else
-         e = p == 0 ? launch_outer<C2C_INVERSE_POSTDIV, true>(xi, t, wi, st, p, numChannels, chanOff, stream)
-                    : launch_outer<C2C_INVERSE_POSTDIV, false>(t, t, wi, st, p, numChannels, nullptr, stream);
-   }
-   return e;
 }
 
 uint32_t large_first_inner_pass(const StageTable &st)
@@ -226,18 +193,20 @@ uint32_t large_first_inner_pass(const StageTable &st)
    return 0;
 }
 
+}  // namespace
+
 hipError_t launch_c2c_large(const float *x, float *y, const float *tw, const StageTable &st, uint32_t numChannels,
-                            int mode, float *scratch, hipStream_t stream, const uint64_t *chanOff)
+                            int mode, float *scratch, hipStream_t stream)
 {
    if (numChannels == 0) return hipSuccess;
    const uint32_t p0 = large_first_inner_pass(st);
    if (p0 == 0) return hipErrorInvalidValue;
    const float2 *xi = (const float2 *) x, *wi = (const float2 *) tw;
    float2       *yo = (float2 *) y, *t = (float2 *) scratch;
-   if (mode == C2C_FORWARD) return run_large<C2C_FORWARD>(xi, yo, wi, st, numChannels, t, p0, chanOff, stream);
+   if (mode == C2C_FORWARD) return run_large<C2C_FORWARD>(xi, yo, wi, st, numChannels, t, p0, stream);
    if (mode == C2C_INVERSE_PRESCALE)
-      return run_large<C2C_INVERSE_PRESCALE>(xi, yo, wi, st, numChannels, t, p0, chanOff, stream);
-   return run_large<C2C_INVERSE_POSTDIV>(xi, yo, wi, st, numChannels, t, p0, chanOff, stream);
+      return run_large<C2C_INVERSE_PRESCALE>(xi, yo, wi, st, numChannels, t, p0, stream);
+   return run_large<C2C_INVERSE_POSTDIV>(xi, yo, wi, st, numChannels, t, p0, stream);
 }
 
 }  // namespace fftlib

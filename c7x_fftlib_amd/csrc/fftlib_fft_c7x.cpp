@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include "fftlib.h"
+#include "fft_fixed_class.h"
 #include "fft_kernels.h"
 #include "fftlib_runtime.h"
 
@@ -89,6 +90,14 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    if (specialized_kernels() && fixed_c2c_supported(p.st.n))
       return launch_c2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st.n, p.numChannels,
                               mode, s);
+   uint32_t n1 = 0, m2 = 0;
+   if (specialized_kernels() && two_pass_split(p.st.n, &n1, &m2)) {
+      void      *scratch = nullptr;
+      hipError_t e       = device_scratch((size_t) p.st.n * p.numChannels * 8u, &scratch);
+      if (e != hipSuccess) return e;
+      return launch_c2c_two_pass((const float *) d.in, (float *) d.out, p.st.n, p.numChannels, mode, (float *) scratch,
+                                 s);
+   }
    if (!fits_lds(p.st)) {
       void      *scratch = nullptr;
       hipError_t e       = device_scratch((size_t) p.st.n * p.numChannels * 8u, &scratch);

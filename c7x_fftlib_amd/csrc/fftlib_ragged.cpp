@@ -12,15 +12,17 @@
 // reference's twiddle tables.
 //
 // Plan: transforms are grouped by length. Lengths up to 16384 become tasks of
-// the multi-length kernel of their octave (fft_fixed_class.h); a longer
-// length first runs its outer radix passes into its own region of a
-// plan-owned scratch buffer, and its sub-blocks then become block-mode tasks
-// of the class kernel of the sub-block length. One direction is therefore:
-// the outer passes of the long lengths (spread over several capture streams),
-// a join, then at most one class-kernel launch per octave (again spread over
-// the streams), a join. Each class kernel walks a persistent list of
-// (task, group) items, largest tasks first, so the whole device stays busy
-// with many short transforms of different lengths at once.
+// the multi-length kernel of their octave (fft_fixed_class.h). A longer length
+// n = n1 * m2 runs in two passes through its own region of a plan-owned
+// scratch buffer: a column task (n1-point DFTs of its m2 columns, twiddled)
+// in the class kernel of n1, then a row task (m2-point DFTs of its n1 rows,
+// written in natural order) in the class kernel of m2. One direction is
+// therefore two phases: phase 0 = the lengths up to 16384 and the column
+// tasks, phase 1 = the row tasks, each at most one class-kernel launch per
+// octave spread over several capture streams, with a join between the
+// phases. Each class kernel walks a persistent list of (task, group) items,
+// largest tasks first, so the whole device stays busy with many short
+// transforms of different lengths at once.
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -43,26 +45,26 @@ struct FFTLIB_GPU_ragged_s {
    struct Group {
       uint32_t          n        = 0;
       uint32_t          count    = 0;
-      bool              large    = false;   // longer than the specialised kernels
-      StageTable        st{};
+      bool              twoPass  = false;   // longer than the class kernels: two passes
+      TwoPassPlan       plan{};
       uint64_t         *dOff     = nullptr; // channel offsets (complex elements)
-      float            *dTw      = nullptr; // twiddle table of length n
-      uint32_t          maxGrid  = 0;       // tuned persistent grid cap (0 = resident slots)
-      size_t            scrOff   = 0;       // large groups: float offset of their scratch region
+      float            *dTw      = nullptr; // twiddle table of length n (one-pass lengths)
+      uint64_t          scrOff   = 0;       // two-pass: complex offset of its scratch region
       double            work     = 0.0;     // n * log2(n) * count, for ordering
    };
    std::vector<Group> groups;
    struct Class {
       int                     cls      = -1;
+      int                     phase    = 0;
       uint32_t                numItems = 0;
       fixedk::ClassTask      *dTasks   = nullptr;
       uint32_t               *dItems   = nullptr;
       uint32_t                maxGrid  = 0;
       double                  work     = 0.0;
    };
-   std::vector<Class> classes;
+   std::vector<Class> classes;   // phase 0 first, then phase 1; by work within a phase
    uint64_t           totalPoints = 0;
-   float             *scratch     = nullptr;
+   float2            *scratch     = nullptr;
    size_t             scratchSize = 0;
    hipStream_t        capture     = nullptr;
    std::vector<hipStream_t> lanes;
@@ -88,9 +90,11 @@ bool smooth235(uint32_t n)
 }
 
 // ---- tuning cache -------------------------------------------------------------
-// Key: the class kernel (kClassKeyBase + octave index) and a power-of-two
-// bucket of its item count; value: the persistent-grid cap and its time.
+// Key: the class-kernel launch (kClassKeyBase + 100 * phase + octave index)
+// and a power-of-two bucket of its item count; value: the persistent-grid cap
+// and its time.
 constexpr uint32_t kClassKeyBase = 100000u;
+uint32_t class_key(const FFTLIB_GPU_ragged_s::Class &c) { return kClassKeyBase + 100u * (uint32_t) c.phase + (uint32_t) c.cls; }
 struct TuneKey {
    uint32_t n, bucket;
    bool     operator<(const TuneKey &o) const { return n != o.n ? n < o.n : bucket < o.bucket; }
@@ -118,17 +122,11 @@ bool tuned(uint32_t n, uint32_t count, uint32_t *maxGrid)
    return true;
 }
 
-hipError_t launch_outer(const FFTLIB_GPU_ragged_s &h, const FFTLIB_GPU_ragged_s::Group &g, const float *in, int mode,
-                        hipStream_t s)
-{
-   return launch_c2c_large_outer(in, g.dTw, g.st, g.count, mode, h.scratch + g.scrOff, s, g.dOff);
-}
-
 hipError_t launch_class(const FFTLIB_GPU_ragged_s &h, const FFTLIB_GPU_ragged_s::Class &c, const float *in, float *out,
                         int mode, uint32_t maxGrid, hipStream_t s)
 {
    fixedk::ClassArgs a{};
-   a.x = (const float2 *) in, a.y = (float2 *) out, a.scratch = (const float2 *) h.scratch, a.mode = mode;
+   a.x = (const float2 *) in, a.y = (float2 *) out, a.scratch = h.scratch, a.mode = mode;
    a.numItems = c.numItems, a.tasks = c.dTasks, a.items = c.dItems;
    return fixed_class_launch(c.cls, a, maxGrid, s);
 }
@@ -178,29 +176,26 @@ hipError_t join_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
    return hipSuccess;
 }
 
-// Enqueues one direction: outer passes of the long lengths, then the class
-// kernels, each phase dealt to the lanes largest first.
+// Enqueues one direction: the class kernels of phase 0, a join, then those
+// of phase 1, each phase dealt to the lanes largest first.
 hipError_t enqueue_direction(FFTLIB_GPU_ragged_s *h, const float *in, float *out, int mode, hipStream_t origin,
                              size_t &ev)
 {
    hipError_t e;
-   size_t     next  = 0;
-   bool       outer = false;
-   for (const auto &g : h->groups) outer = outer || g.large;
-   if (outer) {
+   for (int phase = 0; phase < 2; ++phase) {
+      bool any = false;
+      for (const auto &c : h->classes) any = any || c.phase == phase;
+      if (!any) continue;
       if ((e = fork_lanes(h, origin, ev)) != hipSuccess) return e;
-      for (const auto &g : h->groups) {
-         if (!g.large) continue;
-         if ((e = launch_outer(*h, g, in, mode, h->lanes[next++ % h->lanes.size()])) != hipSuccess) return e;
+      size_t next = 0;
+      for (const auto &c : h->classes) {
+         if (c.phase != phase) continue;
+         if ((e = launch_class(*h, c, in, out, mode, c.maxGrid, h->lanes[next++ % h->lanes.size()])) != hipSuccess)
+            return e;
       }
       if ((e = join_lanes(h, origin, ev)) != hipSuccess) return e;
    }
-   if ((e = fork_lanes(h, origin, ev)) != hipSuccess) return e;
-   next = 0;
-   for (const auto &c : h->classes)
-      if ((e = launch_class(*h, c, in, out, mode, c.maxGrid, h->lanes[next++ % h->lanes.size()])) != hipSuccess)
-         return e;
-   return join_lanes(h, origin, ev);
+   return hipSuccess;
 }
 
 hipError_t capture_graph(FFTLIB_GPU_ragged_s *h, const float *x, float *y, float *z, int dir)
@@ -243,42 +238,38 @@ hipError_t order_after_caller(FFTLIB_GPU_ragged_s *h)
 }
 
 // Class-kernel tasks and items of the plan: one task per length up to 16384
-// and one block-mode task per longer length (its sub-blocks), grouped by the
-// octave of the task length; items (task << 20 | group) largest task first.
+// (phase 0) and a column task (phase 0) plus a row task (phase 1) per longer
+// length, grouped by phase and by the octave of the task length; items
+// (task << 20 | group) largest task first.
 hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
 {
-   std::map<int, std::vector<fixedk::ClassTask>> tasks;
-   std::map<int, double>                         work;
+   std::map<std::pair<int, int>, std::vector<fixedk::ClassTask>> tasks;   // (phase, class)
+   std::map<std::pair<int, int>, double>                         work;
    for (const auto &g : h->groups) {
-      fixedk::ClassTask k{};
-      if (!g.large) {
-         k.n = g.n, k.numChannels = g.count, k.tw = (const float2 *) g.dTw, k.inOff = g.dOff, k.outOff = g.dOff;
+      std::vector<std::pair<int, fixedk::ClassTask>> ks;
+      if (!g.twoPass) {
+         fixedk::ClassTask k{};
+         k.n = g.n, k.numChannels = g.count, k.layout = fixedk::LAYOUT_PLAIN, k.tw = (const float2 *) g.dTw;
+         k.inOff = g.dOff, k.outOff = g.dOff;
+         k.numGroups = (k.numChannels + class_group_size(k.n) - 1u) / class_group_size(k.n);
+         ks.push_back({0, k});
       }
       else {
-         const uint32_t p0 = large_first_inner_pass(g.st);
-         if (p0 == 0) return hipErrorInvalidValue;
-         const uint32_t first = g.st.passFirst[p0];
-         const uint32_t m2    = g.st.stageM[first];
-         k.n = m2, k.nBlocks = g.n / m2, k.numChannels = g.count * k.nBlocks;
-         k.fromScratch = 1, k.scratchOff = g.scrOff / 2u;
-         k.tw = (const float2 *) g.dTw + g.st.twOff[first];
-         k.inOff = nullptr, k.outOff = g.dOff;
-         k.fullN = g.n, k.fullPow2 = g.st.pow2, k.fullLog2 = g.st.log2n, k.fullDigits = g.st.nDigits;
-         k.fullDigitCode = fixedk::pack_digits(g.st.digit, g.st.nDigits);
+         ks.push_back({0, two_pass_task(g.plan, fixedk::LAYOUT_COLUMNS, g.count, g.dOff, g.scrOff)});
+         ks.push_back({1, two_pass_task(g.plan, fixedk::LAYOUT_ROWS, g.count, g.dOff, g.scrOff)});
       }
-      const int cls = fixed_class_of(k.n);
-      if (cls < 0) return hipErrorInvalidValue;
-      uint32_t T = 1;
-      while (T < 256u && T * 16u < k.n) T *= 2u;
-      const uint32_t fpb = 256u / T;
-      k.numGroups        = (k.numChannels + fpb - 1u) / fpb;
-      tasks[cls].push_back(k);
-      work[cls] += g.work;
+      for (const auto &pk : ks) {
+         const int cls = fixed_class_of(pk.second.n);
+         if (cls < 0) return hipErrorInvalidValue;
+         tasks[{pk.first, cls}].push_back(pk.second);
+         work[{pk.first, cls}] += g.work / (double) ks.size();
+      }
    }
    for (auto &kv : tasks) {
       FFTLIB_GPU_ragged_s::Class c;
-      c.cls  = kv.first;
-      c.work = work[kv.first];
+      c.phase = kv.first.first;
+      c.cls   = kv.first.second;
+      c.work  = work[kv.first];
       std::vector<uint32_t> items;
       for (uint32_t t = 0; t < kv.second.size(); ++t) {
          if (kv.second[t].numGroups >= (1u << fixedk::kClassGroupBits) || t >= 4096u) return hipErrorInvalidValue;
@@ -294,10 +285,12 @@ hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
           (e = hipMemcpy(h->classes.back().dItems, items.data(), items.size() * sizeof(uint32_t),
                          hipMemcpyHostToDevice)) != hipSuccess)
          return e;
-      (void) tuned(kClassKeyBase + (uint32_t) c.cls, c.numItems, &h->classes.back().maxGrid);
+      (void) tuned(class_key(h->classes.back()), c.numItems, &h->classes.back().maxGrid);
    }
    std::sort(h->classes.begin(), h->classes.end(),
-             [](const FFTLIB_GPU_ragged_s::Class &a, const FFTLIB_GPU_ragged_s::Class &b) { return a.work > b.work; });
+             [](const FFTLIB_GPU_ragged_s::Class &a, const FFTLIB_GPU_ragged_s::Class &b) {
+                return a.phase != b.phase ? a.phase < b.phase : a.work > b.work;
+             });
    return hipSuccess;
 }
 
@@ -326,25 +319,27 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       FFTLIB_GPU_ragged_s::Group g;
       g.n     = kv.first;
       g.count = (uint32_t) kv.second.size();
-      if (!build_stage_table(g.n, &g.st)) {
-         release(h);
-         delete h;
-         return FFTLIB_ERR_INVALID_DIMENSION;
-      }
-      // lengths up to 16384 are class-kernel tasks; longer ones run outer
-      // passes into their own scratch region first
-      g.large = fixed_class_of(g.n) < 0;
-      if (g.large) {
-         g.scrOff = scr / sizeof(float);
+      // lengths up to 16384 are class-kernel tasks; longer ones run two
+      // passes through their own scratch region
+      g.twoPass = fixed_class_of(g.n) < 0;
+      if (g.twoPass) {
+         if ((e = two_pass_plan(g.n, &g.plan)) != hipSuccess) {
+            release(h);
+            delete h;
+            return fail("ragged two-pass tables", e);
+         }
+         g.scrOff = scr / sizeof(float2);
          scr += ((size_t) g.n * g.count * 8u + 255u) & ~(size_t) 255u;
       }
-      std::vector<float> tw(2 * (size_t) g.n);
-      FFTLIB_tw_gen_c2c(tw.data(), (int32_t) g.n);
-      if ((e = hipMalloc((void **) &g.dOff, kv.second.size() * sizeof(uint64_t))) != hipSuccess ||
-          (e = hipMemcpy(g.dOff, kv.second.data(), kv.second.size() * sizeof(uint64_t), hipMemcpyHostToDevice)) !=
-             hipSuccess ||
-          (e = hipMalloc((void **) &g.dTw, tw.size() * sizeof(float))) != hipSuccess ||
-          (e = hipMemcpy(g.dTw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
+      if ((e = hipMalloc((void **) &g.dOff, kv.second.size() * sizeof(uint64_t))) == hipSuccess)
+         e = hipMemcpy(g.dOff, kv.second.data(), kv.second.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+      if (e == hipSuccess && !g.twoPass) {
+         std::vector<float> tw(2 * (size_t) g.n);
+         FFTLIB_tw_gen_c2c(tw.data(), (int32_t) g.n);
+         if ((e = hipMalloc((void **) &g.dTw, tw.size() * sizeof(float))) == hipSuccess)
+            e = hipMemcpy(g.dTw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice);
+      }
+      if (e != hipSuccess) {
          h->groups.push_back(g);
          release(h);
          delete h;
@@ -454,9 +449,8 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    if ((e = hipEventCreate(&t1)) != hipSuccess) return fail("hipEventCreate", e);
    hipStream_t s = handle->capture;
    if ((e = order_after_caller(handle)) != hipSuccess) return fail("ragged stream ordering", e);
-   // the block-mode tasks read the outer passes' results from the scratch
-   for (const auto &g : handle->groups)
-      if (g.large && e == hipSuccess) e = launch_outer(*handle, g, pX, C2C_FORWARD, s);
+   // phase-0 launches precede phase-1 ones, so the row tasks read column
+   // results of the same input
    for (auto &c : handle->classes) {
       if (e != hipSuccess) break;
       // candidates: resident slots (0), then caps at multiples of the CU count
@@ -477,7 +471,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
       if (e != hipSuccess) break;
       c.maxGrid = bestG;
       std::lock_guard<std::mutex> tl(g_tuneMu);
-      g_tune[TuneKey{kClassKeyBase + (uint32_t) c.cls, bucket_of(c.numItems)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
+      g_tune[TuneKey{class_key(c), bucket_of(c.numItems)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
    }
    (void) hipEventDestroy(t0);
    (void) hipEventDestroy(t1);

@@ -133,7 +133,8 @@ def test_ragged_tuning_keeps_results(tmp_path):
     text = open(path, "rb").read().decode()
     assert text.startswith("fftlib-amd-tune 1\n")
     keys = {int(line.split()[0]) for line in text.strip().splitlines()[1:]}
-    # octave kernels of 250 (128, 256], 960 (512, 1024], 4096 (2048, 4096]
-    # and of 40000's sub-blocks (1600: (1024, 2048])
-    assert {100000 + 4, 100000 + 6, 100000 + 7, 100000 + 8} <= keys
+    # phase 0: octave kernels of 250 and of 40000's columns (200; both in
+    # (128, 256]), 960 (512, 1024], 4096 (2048, 4096]; phase 1: 40000's rows
+    # (200)
+    assert {100000 + 4, 100000 + 6, 100000 + 8, 100000 + 100 + 4} <= keys
     rg.close()
