@@ -222,7 +222,7 @@ class Workload:
                        "transforms": int(len(lengths)), "distinct_lengths": int(len(np.unique(lengths))),
                        "points": total, "tuned": not self.args.no_tune,
                        "input": "complex64 uniform[-1,1) Philox seed 10001"}
-        self.kernel_name = "ragged graph (specialised C2C kernels <= 16384 points, outer-pass path above)"
+        self.kernel_name = "ragged graph (class kernels: lengths <= 16384 in one pass, longer lengths in a column and a row pass)"
         xp, yp, zp, h = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(), self.h
         fn = lib.FFTLIB_GPU_raggedExec
 
