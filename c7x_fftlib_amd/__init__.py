@@ -31,8 +31,10 @@ FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES = 8
 FFTLIB_ERR_INVALID_OUTMODE = 14
 
 FFTLIB_INT16 = 1
+FFTLIB_INT32 = 2
 FFTLIB_UINT32 = 8
 FFTLIB_FLOAT32 = 12
+FFTLIB_ERR_INVALID_SHIFT = 12
 
 FFTLIB_FUNCTION_NATC = 0
 FFTLIB_FUNCTION_OPTIMIZED = 1
@@ -57,6 +59,25 @@ class BufParams1D(ctypes.Structure):
     """FFTLIB_bufParams1D_t: dim_x counts elements (2 per complex point)."""
     _fields_ = [("data_type", ctypes.c_uint32), ("dim_x", ctypes.c_uint32)]
 
+
+class MmaInitArgs(ctypes.Structure):
+    """InitArgs of FFTLIB_FFT_fft_ixX_cxX_oxX and FFTLIB_FFT_highRadixDecompositions_ixX_cxX_oxX
+    (five shifts); the dftSmall / dftLarge InitArgs are the same with one shift."""
+    _fields_ = [("funcStyle", ctypes.c_int8), ("fftSize", ctypes.c_uint32), ("batchSize", ctypes.c_uint32),
+                ("interleave", ctypes.c_uint32), ("shiftVector", ctypes.c_uint32 * 5)]
+
+
+class MmaInitArgs1(ctypes.Structure):
+    _fields_ = [("funcStyle", ctypes.c_int8), ("fftSize", ctypes.c_uint32), ("batchSize", ctypes.c_uint32),
+                ("interleave", ctypes.c_uint32), ("shiftVector", ctypes.c_uint32 * 1)]
+
+
+MMA_KERNELS = {  # alias -> (C name prefix, InitArgs type)
+    "fft": ("FFTLIB_FFT_fft_ixX_cxX_oxX", MmaInitArgs),
+    "dftsmall": ("FFTLIB_FFT_dftSmall_ixX_cxX_oxX", MmaInitArgs1),
+    "dftlarge": ("FFTLIB_FFT_dftLarge_ixX_cxX_oxX", MmaInitArgs1),
+    "highradix": ("FFTLIB_FFT_highRadixDecompositions_ixX_cxX_oxX", MmaInitArgs),
+}
 
 _P = ctypes.c_void_p
 _BP = ctypes.POINTER(BufParams1D)
@@ -137,6 +158,14 @@ def _declare(lib):
             fn = getattr(lib, name + suffix)
             fn.restype = _S
             fn.argtypes = sig
+    for prefix, _args in MMA_KERNELS.values():
+        for suffix, res, argt in (("_getHandleSize", ctypes.c_int32, [_P]), ("_getSizes", _S, [_P, _BP, _BP, _BP]),
+                                  ("_twGen", _S, [_P, _P, _BP]), ("_init", _S, [_P, _BP, _BP, _BP, _P]),
+                                  ("_init_checkParams", _S, [_P, _BP, _BP, _BP, _P]),
+                                  ("_exec", _S, [_P, _P, _P, _P]), ("_exec_checkParams", _S, [_P, _P, _P, _P])):
+            fn = getattr(lib, prefix + suffix)
+            fn.restype = res
+            fn.argtypes = argt
     for name, (res, args) in _EXTRA.items():
         fn = getattr(lib, name, None)
         if fn is not None:

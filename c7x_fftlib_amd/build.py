@@ -26,7 +26,7 @@ SOURCES = [
     "fft_fixed_class3.hip",
     "fft_large.hip",
     "fft_mfma.hip",
-    "fft_ragged.hip",
+    "fft_mma.hip",
     "fftlib_runtime.cpp",
     "fftlib_fft_c7x.cpp",
     "fftlib_ragged.cpp",

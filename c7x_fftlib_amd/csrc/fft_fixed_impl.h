@@ -1,7 +1,8 @@
 // Length-specialised gfx950 kernels (templates): see fft_fixed.hip for the
-// design. Included by fft_fixed.hip and by the translation units that
-// instantiate further lengths (fft_fixed_ext*.hip), so the instantiations
-// compile in parallel.
+// design. Included by fft_fixed.hip (dedicated kernels, two-pass launches)
+// and by the translation units that instantiate the multi-length class
+// kernels (fft_fixed_class0..3.hip), so the instantiations compile in
+// parallel.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -103,10 +103,16 @@ static FFTLIB_STATUS fail(const char *what, hipError_t err)
 
 FFTLIB_STATUS run_on_gpu(const Plan &plan, const CallBuffers &b, LaunchFn launch)
 {
+   return run_buffers(plan.style, b, [&](const DeviceBuffers &d, hipStream_t s) { return launch(plan, d, s); });
+}
+
+FFTLIB_STATUS run_buffers(uint32_t style, const CallBuffers &b,
+                          const std::function<hipError_t(const DeviceBuffers &, hipStream_t)> &launch)
+{
    hipStream_t stream = current_stream();
-   if (plan.style == FFTLIB_FUNCTION_GPU) {
+   if (style == FFTLIB_FUNCTION_GPU) {
       DeviceBuffers d{b.in.ptr, b.tw.ptr, b.aux0.ptr, b.out};
-      hipError_t    e = launch(plan, d, stream);
+      hipError_t    e = launch(d, stream);
       if (e != hipSuccess) return fail("kernel launch", e);
       if (synchronous()) {
          e = hipStreamSynchronize(stream);
@@ -129,7 +135,7 @@ FFTLIB_STATUS run_on_gpu(const Plan &plan, const CallBuffers &b, LaunchFn launch
    if ((e = ensure(3, b.outBytes)) != hipSuccess) return fail("hipMalloc (staging)", e);
    DeviceBuffers d{b.in.ptr ? s.buf[0] : nullptr, b.tw.ptr ? s.buf[1] : nullptr, b.aux0.ptr ? s.buf[2] : nullptr,
                    s.buf[3]};
-   if ((e = launch(plan, d, stream)) != hipSuccess) return fail("kernel launch", e);
+   if ((e = launch(d, stream)) != hipSuccess) return fail("kernel launch", e);
    if ((e = hipMemcpyAsync(b.out, s.buf[3], b.outBytes, hipMemcpyDefault, stream)) != hipSuccess)
       return fail("hipMemcpyAsync (to host)", e);
    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail("hipStreamSynchronize", e);

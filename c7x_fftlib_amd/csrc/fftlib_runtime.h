@@ -7,6 +7,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "FFTLIB_types.h"
 #include "fft_plan.h"
 
@@ -41,6 +43,12 @@ using LaunchFn = hipError_t (*)(const Plan &plan, const DeviceBuffers &d, hipStr
 // FFTLIB_FUNCTION_GPU uses them in place; FFTLIB_FUNCTION_OPTIMIZED stages
 // them through device memory. Returns a FFTLIB status and records HIP errors.
 FFTLIB_STATUS run_on_gpu(const Plan &plan, const CallBuffers &b, LaunchFn launch);
+
+// The same for any launcher: `style` FFTLIB_FUNCTION_GPU uses the buffers in
+// place, FFTLIB_FUNCTION_OPTIMIZED stages them (the staged input buffer is
+// library memory the launcher may overwrite).
+FFTLIB_STATUS run_buffers(uint32_t style, const CallBuffers &b,
+                          const std::function<hipError_t(const DeviceBuffers &, hipStream_t)> &launch);
 
 FFTLIB_FUNCTION_STYLE current_style();
 hipStream_t           current_stream();

@@ -44,9 +44,10 @@ FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable);
  * other lengths always use FFTLIB_C2C_VARIANT_FP32.
  *   FFTLIB_C2C_VARIANT_FP32   (0, default) mixed-radix fp32, the reference's
  *                             algorithm and twiddle table;
- *   FFTLIB_C2C_VARIANT_BF16X3 (1) radix-15 and radix-64 DFT stages as MFMA
+ *   FFTLIB_C2C_VARIANT_BF16X3 (1) radix-15 and radix-16 DFT stages as MFMA
  *                             GEMMs on bf16 hi+mid+lo operand parts (6
- *                             products), FP32 accumulation;
+ *                             products), FP32 accumulation, then an fp32
+ *                             radix-4 stage;
  *   FFTLIB_C2C_VARIANT_FP16X2 (2) same with fp16 hi+lo parts (3 products);
  *   FFTLIB_C2C_VARIANT_BF16X2 (3) same with bf16 hi+lo parts (3 products).
  * The MFMA variants generate their own DFT matrices and do not read pW.
@@ -83,7 +84,12 @@ int32_t       FFTLIB_GPU_getC2CVariant(void);
  * process-wide cache that later plans with the same kernels and work sizes
  * use; FFTLIB_GPU_tuneSave / FFTLIB_GPU_tuneLoad write and read that cache as
  * text ("fftlib-amd-tune 1", then one "key item-count-bucket grid-cap
- * microseconds" line per entry, key = 100000 + octave index). */
+ * microseconds" line per entry, key = 100000 + 100 * phase + octave index;
+ * phase 0 = lengths up to 16384 and the column pass of longer lengths,
+ * phase 1 = the row pass of longer lengths).
+ *
+ * A plan owns device memory: offset and twiddle tables per length, a scratch
+ * buffer of 8 bytes per point of the lengths above 16384, and its graph. */
 typedef struct FFTLIB_GPU_ragged_s *FFTLIB_GPU_raggedHandle;
 FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uint32_t *lengths, uint32_t count);
 FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB_F32 *pX, FFTLIB_F32 *pY,

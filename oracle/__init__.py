@@ -190,3 +190,56 @@ def raw_threads_c2c(xin, w, y, n, b, inverse, threads):
     return load().oracle_fft1dBatched_c2c_threads(_ptr(xin), _ptr(w), _ptr(y), ctypes.c_uint32(n),
                                                   ctypes.c_uint32(b), ctypes.c_int(int(inverse)),
                                                   ctypes.c_int(threads))
+
+
+# --- fft_c7xmma kernels (fixed-point DFT as matrix products) -------------------
+MMA_SIZE = {1: 32, 2: 16}          # FFTLIB_INT16 -> 32 elements per MMA row, FFTLIB_INT32 -> 16
+_NP = {1: np.int16, 2: np.int32}
+_SFX = {1: "i16", 2: "i32"}
+
+
+def mma_variant(fft, batch, interleave, dtype):
+    """0 dftSmall, 1 dftLarge, 2 highRadixDecompositions (the coordinator's choice)."""
+    return load().oracle_mma_variant(ctypes.c_uint32(fft), ctypes.c_uint32(batch), ctypes.c_uint32(interleave),
+                                     ctypes.c_uint32(MMA_SIZE[dtype]))
+
+
+def mma_tables(kind, fft, batch, interleave, dtype):
+    """Caller table of kernel `kind` ('dftsmall', 'dftlarge', 'highradix'), and
+    for highradix also the internal decomposition matrices (w, w1)."""
+    lib, t, mma, sfx = load(), _NP[dtype], MMA_SIZE[dtype], _SFX[dtype]
+    if kind == "dftsmall":
+        w = np.zeros(mma * mma, t)
+        getattr(lib, "oracle_mma_dftsmall_tw_" + sfx)(_ptr(w), ctypes.c_uint32(fft), ctypes.c_uint32(batch),
+                                                     ctypes.c_uint32(interleave))
+        return w, None
+    if kind == "dftlarge":
+        w = np.zeros(4 * fft * fft, t)
+        getattr(lib, "oracle_mma_dftlarge_tw_" + sfx)(_ptr(w), ctypes.c_uint32(fft), ctypes.c_uint32(interleave))
+        return w, None
+    w = np.zeros(lib.oracle_mma_hr_wsize(ctypes.c_uint32(fft), ctypes.c_uint32(mma)), t)
+    w1 = np.zeros(mma * mma, t)
+    getattr(lib, "oracle_mma_hr_tw_" + sfx)(_ptr(w1), _ptr(w), ctypes.c_uint32(fft))
+    return w, w1
+
+
+def mma(kind, x, fft, batch, interleave, shifts, dtype):
+    """Natural-C result of kernel `kind` ('fft' = the coordinator's choice) on
+    x (2*fft*batch values of the data type); returns the output array."""
+    lib, t, mma_n, sfx = load(), _NP[dtype], MMA_SIZE[dtype], _SFX[dtype]
+    if kind == "fft":
+        kind = ("dftsmall", "dftlarge", "highradix")[mma_variant(fft, batch, interleave, dtype)]
+    xin = np.ascontiguousarray(x, t).reshape(-1).copy()
+    y = np.zeros_like(xin)
+    w, w1 = mma_tables(kind, fft, batch, interleave, dtype)
+    sh = np.zeros(5, np.uint32)
+    sh[:len(shifts)] = shifts
+    if kind in ("dftsmall", "dftlarge"):
+        ldw = mma_n if kind == "dftsmall" else 2 * fft
+        getattr(lib, "oracle_mma_matrix_" + sfx)(_ptr(xin), _ptr(w), _ptr(y), ctypes.c_uint32(batch),
+                                                 ctypes.c_uint32(2 * fft), ctypes.c_uint32(ldw),
+                                                 ctypes.c_uint32(int(sh[0])))
+    else:
+        getattr(lib, "oracle_mma_hr_exec_" + sfx)(_ptr(xin), _ptr(w1), _ptr(w), _ptr(y), ctypes.c_uint32(fft),
+                                                  ctypes.c_uint32(batch), _ptr(sh), ctypes.c_uint32(interleave))
+    return y

@@ -80,7 +80,7 @@ class _Mem:
 
     def empty(self, count, dtype):
         if self.style == "gpu":
-            tdt = {np.float32: self.torch.float32, np.int16: self.torch.int16}[dtype]
+            tdt = {np.float32: self.torch.float32, np.int16: self.torch.int16, np.int32: self.torch.int32}[dtype]
             t = self.torch.full((count,), -7, dtype=tdt, device="cuda:0")
             return t, t.data_ptr()
         a = np.full(count, -7, dtype)
@@ -222,3 +222,46 @@ def rel_l2(a, b):
     b = np.asarray(b, np.complex128)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def mma(alias, x, fft, batch, interleave, shifts, dtype, style="gpu"):
+    """Fixed-point DFT kernel `alias` ('fft', 'dftsmall', 'dftlarge',
+    'highradix') on x (2*fft*batch values of FFTLIB_INT16 / FFTLIB_INT32
+    `dtype`), called as the reference driver does (test/fft_c7xmma/<kernel>/
+    <kernel>_d.c): getSizes, twGen, init_checkParams, init, exec_checkParams,
+    exec. Returns (status, output array)."""
+    F_ = F
+    l = lib()
+    l.FFTLIB_GPU_setSynchronous(1)
+    prefix, args_t = F_.MMA_KERNELS[alias]
+    a = args_t()
+    a.funcStyle = F_.FFTLIB_FUNCTION_GPU if style == "gpu" else F_.FFTLIB_FUNCTION_OPTIMIZED
+    a.fftSize, a.batchSize, a.interleave = fft, batch, interleave
+    for i in range(len(a.shiftVector)):
+        a.shiftVector[i] = int(shifts[i])
+    t = np.int16 if dtype == F_.FFTLIB_INT16 else np.int32
+    bx, bw, by = F_.BufParams1D(dtype, 0), F_.BufParams1D(dtype, 0), F_.BufParams1D(dtype, 0)
+    st = getattr(l, prefix + "_getSizes")(ctypes.byref(a), ctypes.byref(bx), ctypes.byref(bw), ctypes.byref(by))
+    if st != 0:
+        return st, None
+    w = np.zeros(bw.dim_x, t)
+    st = getattr(l, prefix + "_twGen")(ctypes.byref(a), w.ctypes.data, ctypes.byref(bw))
+    if st != 0:
+        return st, None
+    handle = ctypes.create_string_buffer(getattr(l, prefix + "_getHandleSize")(ctypes.byref(a)))
+    st = getattr(l, prefix + "_init_checkParams")(handle, ctypes.byref(bx), ctypes.byref(bw), ctypes.byref(by),
+                                                    ctypes.byref(a))
+    if st != 0:
+        return st, None
+    st = getattr(l, prefix + "_init")(handle, ctypes.byref(bx), ctypes.byref(bw), ctypes.byref(by), ctypes.byref(a))
+    if st != 0:
+        return st, None
+    m = _Mem(style)
+    xo, xp = m.put(np.ascontiguousarray(x, t).reshape(-1))
+    wo, wp = m.put(w)
+    yo, yp = m.empty(by.dim_x, t)
+    st = getattr(l, prefix + "_exec_checkParams")(handle, xp, wp, yp)
+    if st != 0:
+        return st, None
+    st = getattr(l, prefix + "_exec")(handle, xp, wp, yp)
+    return st, m.get(yo)

@@ -263,9 +263,11 @@ def test_c2c_variant_other_lengths_use_fp32(c2c_variant):
     assert np.array_equal(y0.view(np.uint32), y1.view(np.uint32))
 
 
-# Complex lengths longer than one workgroup's LDS: outer radix passes over HBM
-# plus sub-blocks finished in LDS.
-LARGE_SIZES = [20000, 30720, 32768, 40000, 50000, 59049, 65536]
+# Complex lengths longer than 16384 points: a column pass and a row pass on
+# the multi-length kernels (n = n1 * m2: 20000 = 160 * 125, 16875 = 135 * 125,
+# 65536 = 256 * 256, 2^20 = 1024 * 1024); with the generic kernels, outer radix
+# passes over HBM plus sub-blocks finished in LDS.
+LARGE_SIZES = [16875, 20000, 30720, 32768, 40000, 50000, 59049, 65536]
 
 
 @pytest.mark.parametrize("kind", ["fft", "ifft", "ifft_single"])
@@ -278,6 +280,23 @@ def test_c2c_large(n, kind):
     assert s == 0, G.F.last_error()
     ref = oracle.c2c(x, n, b, inverse=kind != "fft", single=single)
     assert G.rel_l2(y, ref) <= REL_TOL
+
+
+def test_c2c_large_two_pass_longest():
+    n = 1 << 20
+    x = cplx(rng(20), 2, n, 1.0)
+    s, y = G.c2c(x, n, 2)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.c2c(x, n, 2)) <= REL_TOL
+
+
+@pytest.mark.parametrize("kind", ["fft", "ifft"])
+@pytest.mark.parametrize("n", [20000, 65536])
+def test_c2c_large_generic(n, kind, generic_kernels):
+    x = cplx(rng(n + 7), 3, n, 1.0)
+    s, y = G.c2c(x, n, 3, inverse=kind != "fft")
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.c2c(x, n, 3, inverse=kind != "fft")) <= REL_TOL
 
 
 def test_real_too_long_is_refused():
