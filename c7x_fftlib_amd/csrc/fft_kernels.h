@@ -54,6 +54,19 @@ hipError_t launch_c2r(const float *x, float *y, const float *tw, const float *sf
 hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uint32_t *shift,
                       const StageTable &st, uint32_t numChannels, hipStream_t stream);
 
+// Real transforms longer than one workgroup's LDS run as a complex FFT of
+// nc = N/2 points plus a separate elementwise pass over HBM through `z`
+// (nc * numChannels complex values, the reference's pXFFT role):
+//   R2C: complex FFT x -> z, then launch_r2c_split z -> y (the split of
+//        launch_r2c: Y[i] = Z[i] A[i] + mirrored Z[nc-i] B[i], Nyquist bin,
+//        outmode 1 conjugate half);
+//   C2R: launch_c2r_presplit x -> z (Z[i] = X[i] A'[i] + V[i] B'[i]), then
+//        the complex inverse FFT z -> y.
+hipError_t launch_r2c_split(const float *z, float *y, const float *sf, uint32_t nc, uint32_t numChannels,
+                            uint32_t outmode, hipStream_t stream);
+hipError_t launch_c2r_presplit(const float *x, float *z, const float *sf, uint32_t nc, uint32_t numChannels,
+                               uint32_t inMode, hipStream_t stream);
+
 // Largest complex length the single-pass LDS kernels accept.
 uint32_t max_lds_points();
 

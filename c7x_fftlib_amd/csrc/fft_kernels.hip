@@ -327,6 +327,77 @@ uint32_t max_lds_bytes()
 // Launch geometry: transforms per workgroup (F) and LDS bytes. The twiddle
 // table is staged in LDS when it fits next to the data, otherwise the stages
 // read it from global memory (L1/L2 resident).
+// Split pass of the long real-to-complex path: z holds the nc-point complex
+// FFT of every channel; one work item per output bin i < nc (the arithmetic
+// of r2c_lds_kernel's store loop).
+__global__ void __launch_bounds__(kThreads) r2c_split_kernel(const float2 *__restrict__ z, float2 *__restrict__ y,
+                                                             const float *__restrict__ sf, uint32_t Nc,
+                                                             uint32_t numChannels, uint32_t outmode)
+{
+   const uint32_t N     = 2u * Nc;
+   const uint32_t nOut  = outmode ? N : Nc + 1u;
+   const size_t   total = (size_t) Nc * numChannels;
+   for (size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+        idx += (size_t) gridDim.x * blockDim.x) {
+      const uint32_t f   = (uint32_t) (idx / Nc);
+      const uint32_t i   = (uint32_t) (idx - (size_t) f * Nc);
+      const float2  *zc  = z + (size_t) f * Nc;
+      float2        *out = y + (size_t) f * nOut;
+      const float2   zi  = zc[i];
+      const float    ar = sf[2u * i], ai = sf[2u * i + 1u];
+      const float    br = sf[2u * N - 2u * i - 2u], bi = sf[2u * N - 2u * i - 1u];
+      float2         t1;
+      if (i == 0u) {
+         t1 = make_float2(zi.x * br + zi.y * bi, zi.x * br - zi.y * bi);
+      }
+      else {
+         const float2 zm = zc[Nc - i];
+         t1              = make_float2(zm.x * br + zm.y * bi, zm.x * bi - zm.y * br);
+      }
+      const float2 v = make_float2(zi.x * ar - zi.y * ai + t1.x, zi.x * ai + zi.y * ar + t1.y);
+      out[i]         = v;
+      if (outmode && i > 0u) out[N - i] = make_float2(v.x, -v.y);
+      if (i == 0u) out[Nc] = make_float2(zi.x - zi.y, 0.0f);
+   }
+}
+
+// Pre-split pass of the long complex-to-real path: Z[i] = X[i] A'[i] +
+// V[i] B'[i] with V = conj(X[nc-i]) (inMode 0) or X[nc+i] (inMode 1), the
+// arithmetic of c2r_lds_kernel's load loop before its conjugation and scale.
+__global__ void __launch_bounds__(kThreads) c2r_presplit_kernel(const float2 *__restrict__ x, float2 *__restrict__ z,
+                                                                const float *__restrict__ sf, uint32_t Nc,
+                                                                uint32_t numChannels, uint32_t inMode)
+{
+   const uint32_t N        = 2u * Nc;
+   const uint32_t inStride = inMode ? N : Nc + 1u;
+   const size_t   total    = (size_t) Nc * numChannels;
+   for (size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+        idx += (size_t) gridDim.x * blockDim.x) {
+      const uint32_t f  = (uint32_t) (idx / Nc);
+      const uint32_t i  = (uint32_t) (idx - (size_t) f * Nc);
+      const float2  *in = x + (size_t) f * inStride;
+      const float2   v0 = in[i];
+      float2         v1;
+      if (inMode) {
+         v1 = in[Nc + i];
+      }
+      else {
+         v1   = in[Nc - i];
+         v1.y = -v1.y;
+      }
+      const float ar = sf[2u * i], ai = sf[2u * i + 1u];
+      const float br = sf[N + 2u * i], bi = sf[N + 2u * i + 1u];
+      z[idx] = make_float2(v0.x * ar - v0.y * ai + (v1.x * br - v1.y * bi),
+                           v0.x * ai + v0.y * ar + (v1.x * bi + v1.y * br));
+   }
+}
+
+uint32_t elementwise_blocks(size_t items)
+{
+   const size_t b = (items + kThreads - 1) / kThreads;
+   return (uint32_t) (b < 65536 ? (b > 0 ? b : 1) : 65536);
+}
+
 struct Geometry {
    uint32_t F;
    size_t   ldsBytes;
@@ -425,6 +496,24 @@ hipError_t launch_c2r(const float *x, float *y, const float *tw, const float *sf
                        : launch(c2r_lds_kernel<true, false>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode);
    return g.twInLds ? launch(c2r_lds_kernel<false, true>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode)
                     : launch(c2r_lds_kernel<false, false>, g, numChannels, stream, xi, yo, wi, sf, st, numChannels, g.F, inMode);
+}
+
+hipError_t launch_r2c_split(const float *z, float *y, const float *sf, uint32_t nc, uint32_t numChannels,
+                            uint32_t outmode, hipStream_t stream)
+{
+   if (numChannels == 0 || nc == 0) return hipSuccess;
+   hipLaunchKernelGGL(r2c_split_kernel, dim3(elementwise_blocks((size_t) nc * numChannels)), dim3(kThreads), 0, stream,
+                      (const float2 *) z, (float2 *) y, sf, nc, numChannels, outmode);
+   return hipGetLastError();
+}
+
+hipError_t launch_c2r_presplit(const float *x, float *z, const float *sf, uint32_t nc, uint32_t numChannels,
+                               uint32_t inMode, hipStream_t stream)
+{
+   if (numChannels == 0 || nc == 0) return hipSuccess;
+   hipLaunchKernelGGL(c2r_presplit_kernel, dim3(elementwise_blocks((size_t) nc * numChannels)), dim3(kThreads), 0,
+                      stream, (const float2 *) x, (float2 *) z, sf, nc, numChannels, inMode);
+   return hipGetLastError();
 }
 
 hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uint32_t *shift,

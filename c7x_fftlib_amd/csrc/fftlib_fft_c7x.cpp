@@ -6,7 +6,8 @@
 //
 // Behaviour beyond the reference: _kernel validates the plan and refuses NULL
 // buffers instead of faulting, and pX is never written (the reference uses it
-// as scratch). The fused real-transform kernels do not write pXFFT.
+// as scratch). The real transforms do not write pXFFT (half lengths that do
+// not fit one workgroup's LDS go through library scratch instead).
 #include <string.h>
 
 #include "fftlib.h"
@@ -58,12 +59,12 @@ FFTLIB_STATUS write_plan(void *pBlock, KernelId kernel, uint32_t numPoints, uint
    return FFTLIB_SUCCESS;
 }
 
-// The fused real-transform and int16 kernels keep a whole transform in one
-// workgroup's LDS; longer ones are refused at _kernel time.
+// The int16 kernels keep a whole transform in one workgroup's LDS; longer
+// ones are refused at _kernel time.
 FFTLIB_STATUS refuse_too_long(const Plan &p)
 {
    if (fits_lds(p.st)) return FFTLIB_SUCCESS;
-   set_last_error_text("transform longer than one workgroup's LDS: only the complex (C2C/IFFT) kernels support it");
+   set_last_error_text("int16 transform longer than one workgroup's LDS");
    return FFTLIB_ERR_NOT_IMPLEMENTED;
 }
 
@@ -79,51 +80,73 @@ FFTLIB_STATUS read_plan(const void *pBlock, KernelId kernel, uint32_t numPoints,
 }
 
 // ---- launchers bound to plans ---------------------------------------------
+// Batched complex FFT of st.n points: the MFMA kernel when selected, the
+// dedicated kernel of the length, two passes on the multi-length kernels for
+// long lengths, else the generic LDS or outer-pass kernels.
+hipError_t run_complex(const StageTable &st, const float *x, float *y, const float *tw, uint32_t numChannels,
+                       int mode, hipStream_t s)
+{
+   const int variant = c2c_variant();
+   if (variant != C2C_VARIANT_FP32 && mfma_c2c_supported(st.n))
+      return launch_c2c_mfma(x, y, st.n, numChannels, mode, variant, s);
+   if (specialized_kernels() && fixed_c2c_supported(st.n))
+      return launch_c2c_fixed(x, y, tw, st.n, numChannels, mode, s);
+   uint32_t n1 = 0, m2 = 0;
+   if (specialized_kernels() && two_pass_split(st.n, &n1, &m2)) {
+      void      *scratch = nullptr;
+      hipError_t e       = device_scratch((size_t) st.n * numChannels * 8u, s, SCRATCH_COMPLEX, &scratch);
+      if (e != hipSuccess) return e;
+      return launch_c2c_two_pass(x, y, st.n, numChannels, mode, (float *) scratch, s);
+   }
+   if (!fits_lds(st)) {
+      void      *scratch = nullptr;
+      hipError_t e       = device_scratch((size_t) st.n * numChannels * 8u, s, SCRATCH_COMPLEX, &scratch);
+      if (e != hipSuccess) return e;
+      return launch_c2c_large(x, y, tw, st, numChannels, mode, (float *) scratch, s);
+   }
+   return launch_c2c(x, y, tw, st, numChannels, mode, s);
+}
+
 hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
    int mode = C2C_FORWARD;
    if (p.kernel == K_IC2C_BATCHED) mode = C2C_INVERSE_PRESCALE;
    if (p.kernel == K_IC2C_SINGLE) mode = C2C_INVERSE_POSTDIV;
-   const int variant = c2c_variant();
-   if (variant != C2C_VARIANT_FP32 && mfma_c2c_supported(p.st.n))
-      return launch_c2c_mfma((const float *) d.in, (float *) d.out, p.st.n, p.numChannels, mode, variant, s);
-   if (specialized_kernels() && fixed_c2c_supported(p.st.n))
-      return launch_c2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st.n, p.numChannels,
-                              mode, s);
-   uint32_t n1 = 0, m2 = 0;
-   if (specialized_kernels() && two_pass_split(p.st.n, &n1, &m2)) {
-      void      *scratch = nullptr;
-      hipError_t e       = device_scratch((size_t) p.st.n * p.numChannels * 8u, &scratch);
-      if (e != hipSuccess) return e;
-      return launch_c2c_two_pass((const float *) d.in, (float *) d.out, p.st.n, p.numChannels, mode, (float *) scratch,
-                                 s);
-   }
-   if (!fits_lds(p.st)) {
-      void      *scratch = nullptr;
-      hipError_t e       = device_scratch((size_t) p.st.n * p.numChannels * 8u, &scratch);
-      if (e != hipSuccess) return e;
-      return launch_c2c_large((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st, p.numChannels, mode,
-                              (float *) scratch, s);
-   }
-   return launch_c2c((const float *) d.in, (float *) d.out, (const float *) d.tw, p.st, p.numChannels, mode, s);
+   return run_complex(p.st, (const float *) d.in, (float *) d.out, (const float *) d.tw, p.numChannels, mode, s);
 }
 
+// Real transforms: one fused kernel (half-length complex FFT in LDS plus the
+// split) when the half length fits one workgroup; otherwise the complex FFT
+// and the split as two passes through library scratch.
 hipError_t run_r2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
+   const float *x = (const float *) d.in, *tw = (const float *) d.tw, *sf = (const float *) d.aux0;
+   float       *y = (float *) d.out;
    if (specialized_kernels() && fixed_real_supported(p.st.n))
-      return launch_r2c_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0,
-                              p.st.n, p.numChannels, p.mode, s);
-   return launch_r2c((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0, p.st,
-                     p.numChannels, p.mode, s);
+      return launch_r2c_fixed(x, y, tw, sf, p.st.n, p.numChannels, p.mode, s);
+   if (fits_lds(p.st)) return launch_r2c(x, y, tw, sf, p.st, p.numChannels, p.mode, s);
+   void      *z = nullptr;
+   hipError_t e = device_scratch((size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
+   if (e == hipSuccess) e = run_complex(p.st, x, (float *) z, tw, p.numChannels, C2C_FORWARD, s);
+   if (e == hipSuccess) e = launch_r2c_split((const float *) z, y, sf, p.st.n, p.numChannels, p.mode, s);
+   return e;
 }
 
 hipError_t run_c2r(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
+   const float *x = (const float *) d.in, *tw = (const float *) d.tw, *sf = (const float *) d.aux0;
+   float       *y       = (float *) d.out;
+   const int    postdiv = p.kernel == K_C2R_SINGLE ? 1 : 0;
    if (specialized_kernels() && fixed_real_supported(p.st.n))
-      return launch_c2r_fixed((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0,
-                              p.st.n, p.numChannels, p.mode, p.kernel == K_C2R_SINGLE ? 1 : 0, s);
-   return launch_c2r((const float *) d.in, (float *) d.out, (const float *) d.tw, (const float *) d.aux0, p.st,
-                     p.numChannels, p.mode, p.kernel == K_C2R_SINGLE ? 1 : 0, s);
+      return launch_c2r_fixed(x, y, tw, sf, p.st.n, p.numChannels, p.mode, postdiv, s);
+   if (fits_lds(p.st)) return launch_c2r(x, y, tw, sf, p.st, p.numChannels, p.mode, postdiv, s);
+   void      *z = nullptr;
+   hipError_t e = device_scratch((size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
+   if (e == hipSuccess) e = launch_c2r_presplit(x, (float *) z, sf, p.st.n, p.numChannels, p.mode, s);
+   if (e == hipSuccess)
+      e = run_complex(p.st, (const float *) z, y, tw, p.numChannels,
+                      postdiv ? C2C_INVERSE_POSTDIV : C2C_INVERSE_PRESCALE, s);
+   return e;
 }
 
 hipError_t run_i16(const Plan &p, const DeviceBuffers &d, hipStream_t s)
@@ -132,96 +155,108 @@ hipError_t run_i16(const Plan &p, const DeviceBuffers &d, hipStream_t s)
                      p.st, p.numChannels, s);
 }
 
-FFTLIB_STATUS exec_c2c(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, FFTLIB_F32 *pY,
-                       uint32_t numPoints, uint32_t numChannels, void *pBlock)
+}  // namespace
+
+// Buffers and launcher of one call of plan p (numChannels and mode as the
+// call gives them). `device`: the buffers are used in place, so they must be
+// aligned for the kernels' vector accesses. The checks run in the order the
+// entry points of each kernel family report them.
+FFTLIB_STATUS fftlib::describe_call(const Plan &p, const void *pX, const void *pW, const void *pAux, void *pY,
+                                    bool device, CallBuffers *b, LaunchFn *launch)
+{
+   const size_t  tw = (size_t) p.st.twCount * 2u;
+   FFTLIB_STATUS st;
+   switch (p.kernel) {
+   case K_C2C_BATCHED:
+   case K_C2C_SINGLE:
+   case K_IC2C_BATCHED:
+   case K_IC2C_SINGLE: {
+      if (pX == nullptr || pW == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+      if (device && (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8)))
+         return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+      const size_t data = (size_t) p.numPoints * p.numChannels * 2u * sizeof(float);
+      *b                = CallBuffers{{pX, data}, {pW, tw * sizeof(float)}, {nullptr, 0}, pY, data};
+      *launch           = run_c2c;
+      return FFTLIB_SUCCESS;
+   }
+   case K_I16_BATCHED:
+   case K_I16_SINGLE: {
+      if (pX == nullptr || pW == nullptr || pY == nullptr || pAux == nullptr) return FFTLIB_ERR_NULL_POINTER;
+      if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
+      if (device && (misaligned(pX, 4) || misaligned(pW, 4) || misaligned(pY, 4) || misaligned(pAux, 4)))
+         return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+      const uint32_t need = radix4_stages(p.st);
+      if (p.numShifts < need) return FFTLIB_ERR_INVALID_DIMENSION;
+      const size_t data = (size_t) p.numPoints * p.numChannels * 2u * sizeof(int16_t);
+      *b = CallBuffers{{pX, data}, {pW, tw * sizeof(int16_t)}, {pAux, (size_t) need * sizeof(uint32_t)}, pY, data};
+      *launch = run_i16;
+      return FFTLIB_SUCCESS;
+   }
+   case K_R2C_BATCHED:
+   case K_R2C_SINGLE:
+   case K_C2R_BATCHED:
+   case K_C2R_SINGLE: {
+      const bool r2c = p.kernel == K_R2C_BATCHED || p.kernel == K_R2C_SINGLE;
+      if (p.mode > 1u) return r2c ? FFTLIB_ERR_INVALID_OUTMODE : FFTLIB_ERR_INVALID_VALUE;
+      if (pX == nullptr || pW == nullptr || pAux == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
+      if (device && (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pAux, 4)))
+         return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
+      const uint32_t nSpec = p.mode ? p.numPoints : p.numPoints / 2u + 1u;
+      const size_t   real  = (size_t) p.numPoints * p.numChannels * sizeof(float);
+      const size_t   spec  = (size_t) nSpec * p.numChannels * 2u * sizeof(float);
+      const IoBuffer sf{pAux, (size_t) p.numPoints * 2u * sizeof(float)};
+      if (r2c) *b = CallBuffers{{pX, real}, {pW, tw * sizeof(float)}, sf, pY, spec};
+      else *b = CallBuffers{{pX, spec}, {pW, tw * sizeof(float)}, sf, pY, real};
+      *launch = r2c ? run_r2c : run_c2r;
+      return FFTLIB_SUCCESS;
+   }
+   default:
+      return FFTLIB_ERR_INVALID_VALUE;
+   }
+}
+
+namespace {
+
+// _kernel of every fft_c7x kernel: the plan of pBlock with the call's
+// channel count (and outmode / inMode where the entry point takes one).
+FFTLIB_STATUS exec_plan(KernelId kernel, uint32_t numPoints, uint32_t numChannels, const uint32_t *mode,
+                        const void *pX, const void *pW, const void *pAux, void *pY, void *pBlock)
 {
    Plan          p;
    FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
    if (st != FFTLIB_SUCCESS) return st;
-   if (pX == nullptr || pW == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
-   if (p.style == FFTLIB_FUNCTION_GPU && (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8)))
-      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
-   p.numChannels     = numChannels;
-   const size_t data = (size_t) numPoints * numChannels * 2u * sizeof(float);
-   CallBuffers  b{{pX, data}, {pW, (size_t) p.st.twCount * 2u * sizeof(float)}, {nullptr, 0}, pY, data};
-   if (data == 0) return FFTLIB_SUCCESS;
-   return run_on_gpu(p, b, run_c2c);
+   p.numChannels = numChannels;
+   if (mode) p.mode = *mode;
+   CallBuffers b;
+   LaunchFn    launch;
+   if ((st = describe_call(p, pX, pW, pAux, pY, p.style == FFTLIB_FUNCTION_GPU, &b, &launch)) != FFTLIB_SUCCESS)
+      return st;
+   if (b.in.bytes == 0 || b.outBytes == 0) return FFTLIB_SUCCESS;
+   return run_on_gpu(p, b, launch);
+}
+
+FFTLIB_STATUS exec_c2c(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, FFTLIB_F32 *pY,
+                       uint32_t numPoints, uint32_t numChannels, void *pBlock)
+{
+   return exec_plan(kernel, numPoints, numChannels, nullptr, pX, pW, nullptr, pY, pBlock);
 }
 
 FFTLIB_STATUS exec_i16(KernelId kernel, const int16_t *pX, const int16_t *pW, int16_t *pY, const uint32_t *pShift,
                        uint32_t numPoints, uint32_t numChannels, void *pBlock)
 {
-   Plan          p;
-   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
-   if (st != FFTLIB_SUCCESS) return st;
-   if (pX == nullptr || pW == nullptr || pY == nullptr || pShift == nullptr) return FFTLIB_ERR_NULL_POINTER;
-   if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
-   if (p.style == FFTLIB_FUNCTION_GPU &&
-       (misaligned(pX, 4) || misaligned(pW, 4) || misaligned(pY, 4) || misaligned(pShift, 4)))
-      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
-   const uint32_t need = radix4_stages(p.st);
-   if (p.numShifts < need) return FFTLIB_ERR_INVALID_DIMENSION;
-   p.numChannels     = numChannels;
-   const size_t data = (size_t) numPoints * numChannels * 2u * sizeof(int16_t);
-   CallBuffers  b{{pX, data},
-                 {pW, (size_t) p.st.twCount * 2u * sizeof(int16_t)},
-                 {pShift, (size_t) need * sizeof(uint32_t)},
-                 pY,
-                 data};
-   if (data == 0) return FFTLIB_SUCCESS;
-   return run_on_gpu(p, b, run_i16);
+   return exec_plan(kernel, numPoints, numChannels, nullptr, pX, pW, pShift, pY, pBlock);
 }
 
 FFTLIB_STATUS exec_r2c(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, const FFTLIB_F32 *pSf,
                        FFTLIB_F32 *pY, uint32_t numPoints, uint32_t numChannels, uint32_t outmode, void *pBlock)
 {
-   Plan          p;
-   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
-   if (st != FFTLIB_SUCCESS) return st;
-   if (outmode > 1u) return FFTLIB_ERR_INVALID_OUTMODE;
-   if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
-   if (pX == nullptr || pW == nullptr || pSf == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
-   if (p.style == FFTLIB_FUNCTION_GPU &&
-       (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pSf, 4)))
-      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
-   p.numChannels       = numChannels;
-   p.mode              = outmode;
-   const uint32_t nOut = outmode ? numPoints : numPoints / 2u + 1u;
-   const size_t   in   = (size_t) numPoints * numChannels * sizeof(float);
-   const size_t   out  = (size_t) nOut * numChannels * 2u * sizeof(float);
-   CallBuffers    b{{pX, in},
-                 {pW, (size_t) p.st.twCount * 2u * sizeof(float)},
-                 {pSf, (size_t) numPoints * 2u * sizeof(float)},
-                 pY,
-                 out};
-   if (in == 0) return FFTLIB_SUCCESS;
-   return run_on_gpu(p, b, run_r2c);
+   return exec_plan(kernel, numPoints, numChannels, &outmode, pX, pW, pSf, pY, pBlock);
 }
 
 FFTLIB_STATUS exec_c2r(KernelId kernel, const FFTLIB_F32 *pX, const FFTLIB_F32 *pW, const FFTLIB_F32 *pSf,
                        FFTLIB_F32 *pY, uint32_t numPoints, uint32_t numChannels, uint32_t inMode, void *pBlock)
 {
-   Plan          p;
-   FFTLIB_STATUS st = read_plan(pBlock, kernel, numPoints, &p);
-   if (st != FFTLIB_SUCCESS) return st;
-   if (inMode > 1u) return FFTLIB_ERR_INVALID_VALUE;
-   if ((st = refuse_too_long(p)) != FFTLIB_SUCCESS) return st;
-   if (pX == nullptr || pW == nullptr || pSf == nullptr || pY == nullptr) return FFTLIB_ERR_NULL_POINTER;
-   if (p.style == FFTLIB_FUNCTION_GPU &&
-       (misaligned(pX, 8) || misaligned(pW, 8) || misaligned(pY, 8) || misaligned(pSf, 4)))
-      return FFTLIB_ERR_NOT_ALIGNED_PTRS_STRIDES;
-   p.numChannels      = numChannels;
-   p.mode             = inMode;
-   const uint32_t nIn = inMode ? numPoints : numPoints / 2u + 1u;
-   const size_t   in  = (size_t) nIn * numChannels * 2u * sizeof(float);
-   const size_t   out = (size_t) numPoints * numChannels * sizeof(float);
-   CallBuffers    b{{pX, in},
-                 {pW, (size_t) p.st.twCount * 2u * sizeof(float)},
-                 {pSf, (size_t) numPoints * 2u * sizeof(float)},
-                 pY,
-                 out};
-   if (out == 0) return FFTLIB_SUCCESS;
-   return run_on_gpu(p, b, run_c2r);
+   return exec_plan(kernel, numPoints, numChannels, &inMode, pX, pW, pSf, pY, pBlock);
 }
 
 // ---- parameter checks (reference _ci checkParams) ---------------------------

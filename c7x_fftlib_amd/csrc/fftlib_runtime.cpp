@@ -8,7 +8,9 @@
 #include <string.h>
 
 #include <atomic>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "fftlib_gpu.h"
 
@@ -66,24 +68,23 @@ void set_last_error(const char *what, hipError_t err)
 
 void set_last_error_text(const char *text) { snprintf(t_lastError, sizeof t_lastError, "%s", text); }
 
-hipError_t device_scratch(size_t bytes, void **ptr)
+hipError_t device_scratch(size_t bytes, hipStream_t stream, int role, void **ptr)
 {
    struct Slot {
       void  *p   = nullptr;
       size_t cap = 0;
    };
-   static std::mutex mu;
-   static Slot       slots[16];
-   int               dev = 0;
-   hipError_t        e   = hipGetDevice(&dev);
+   static std::mutex                                          mu;
+   static std::map<std::tuple<int, hipStream_t, int>, Slot>   slots;
+   int                                                        dev = 0;
+   hipError_t                                                 e   = hipGetDevice(&dev);
    if (e != hipSuccess) return e;
-   if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
    std::lock_guard<std::mutex> lock(mu);
-   Slot &s = slots[dev];
+   Slot &s = slots[std::make_tuple(dev, stream, role)];
    if (bytes > s.cap) {
       if (s.p) {
-         // a kernel of an earlier call may still read the old buffer
-         if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+         // a kernel of an earlier call on this stream may still read it
+         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
          if ((e = hipFree(s.p)) != hipSuccess) return e;
          s.p = nullptr, s.cap = 0;
       }

@@ -58,9 +58,20 @@ int                   c2c_variant();
 void                  set_last_error(const char *what, hipError_t err);
 void                  set_last_error_text(const char *text);
 
-// Library-owned device scratch of at least `bytes` on the current device
-// (grow-only, kept until the process ends). Calls that use it on one device
-// must not overlap in time; the GPU controls document this.
-hipError_t device_scratch(size_t bytes, void **ptr);
+// Buffers and launcher of one call of an fft_c7x plan (numChannels and mode
+// as the call gives them). `device`: the buffers are used in place and must
+// be aligned for the kernels. Returns the status the kernel's _kernel reports
+// for bad buffers.
+FFTLIB_STATUS describe_call(const Plan &p, const void *pX, const void *pW, const void *pAux, void *pY, bool device,
+                            CallBuffers *b, LaunchFn *launch);
+
+// Library-owned device scratch of at least `bytes` for work enqueued on
+// `stream` of the current device: one grow-only buffer per (device, stream,
+// role), kept until the process ends, so calls on different streams never
+// share one. Growing it waits for the stream's earlier work. Roles: the
+// intermediate of long complex transforms, and the complex half-length
+// spectrum of long real transforms (which itself runs a long complex one).
+enum ScratchRole : int { SCRATCH_COMPLEX = 0, SCRATCH_REAL = 1 };
+hipError_t device_scratch(size_t bytes, hipStream_t stream, int role, void **ptr);
 
 }  // namespace fftlib

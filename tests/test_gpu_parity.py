@@ -299,8 +299,47 @@ def test_c2c_large_generic(n, kind, generic_kernels):
     assert G.rel_l2(y, oracle.c2c(x, n, 3, inverse=kind != "fft")) <= REL_TOL
 
 
-def test_real_too_long_is_refused():
-    n = 65536
-    x = rng(3).uniform(-1, 1, (1, n)).astype(np.float32)
-    s, _ = G.r2c(x, n, 1, outmode=0)
+# Real transforms whose half length does not fit one workgroup's LDS: the
+# half-length complex FFT and the split run as two passes through library
+# scratch (the reference's test lists go up to N = 62500 batched and 156250
+# single-channel). Half lengths: 20000, 31250, 32768, 65536.
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n,b", [(40000, 3), (62500, 2), (65536, 2), (131072, 1)])
+def test_real_long_batched(n, b, mode):
+    x = rng(n + mode).uniform(-1, 1, (b, n)).astype(np.float32)
+    s, y = G.r2c(x, n, b, outmode=mode)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.r2c(x, n, b, outmode=mode)) <= REL_TOL
+    s, z = G.c2r(y, n, b, inmode=mode)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(z, oracle.c2r(y, n, b, inmode=mode)) <= REL_TOL
+
+
+@pytest.mark.parametrize("n", [65610, 156250])
+def test_real_long_single(n):
+    x = rng(n).uniform(-128, 128, (1, n)).astype(np.float32)
+    s, y = G.r2c(x, n, 1, single=True)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.r2c(x, n, 1, single=True)) <= REL_TOL
+    s, z = G.c2r(y, n, 1, single=True)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(z, oracle.c2r(y, n, 1, single=True)) <= REL_TOL
+
+
+def test_real_long_generic(generic_kernels):
+    """The same split passes around the generic outer-pass complex kernels."""
+    n, b = 40000, 2
+    x = rng(9).uniform(-1, 1, (b, n)).astype(np.float32)
+    s, y = G.r2c(x, n, b, outmode=0)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(y, oracle.r2c(x, n, b, outmode=0)) <= REL_TOL
+    s, z = G.c2r(y, n, b, inmode=0)
+    assert s == 0, G.F.last_error()
+    assert G.rel_l2(z, oracle.c2r(y, n, b, inmode=0)) <= REL_TOL
+
+
+def test_i16_too_long_is_refused():
+    n = 32768
+    x = np.zeros((1, n, 2), np.int16)
+    s, _ = G.i16(x, n, 1, [0] * 8, single=True)
     assert s == G.F.FFTLIB_ERR_NOT_IMPLEMENTED
