@@ -7,7 +7,13 @@ _U32 = ctypes.c_uint32
 
 # name -> (restype, argtypes); entries are declared only when the library
 # exports them.
-DECLS = {}
+DECLS = {
+    "FFTLIB_GPU_kernelAsync": (_S, [_P, _P, _P, _P, _P, _P]),
+    "FFTLIB_GPU_pipelineCreate": (_S, [_P, _U32]),
+    "FFTLIB_GPU_pipelineSubmit": (_S, [_P, _P, _P, _P, _P, _P]),
+    "FFTLIB_GPU_pipelineWait": (_S, [_P]),
+    "FFTLIB_GPU_pipelineDestroy": (_S, [_P]),
+}
 
 
 def declare(lib):

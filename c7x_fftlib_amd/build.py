@@ -31,6 +31,7 @@ SOURCES = [
     "fftlib_fft_c7x.cpp",
     "fftlib_ragged.cpp",
     "fftlib_mma.cpp",
+    "fftlib_async.cpp",
 ]
 
 ARCH = "gfx950"

@@ -100,6 +100,44 @@ uint64_t      FFTLIB_GPU_raggedPoints(FFTLIB_GPU_raggedHandle handle);
 FFTLIB_STATUS FFTLIB_GPU_tuneSave(const char *path);
 FFTLIB_STATUS FFTLIB_GPU_tuneLoad(const char *path);
 
+/* Asynchronous calls (no counterpart in the reference, whose _kernel entry
+ * points are synchronous; for applications that overlap FFTs with other GPU
+ * work or keep their data in host memory).
+ *
+ * FFTLIB_GPU_kernelAsync runs the transform an fft_c7x _init call planned in
+ * pBlock (any of the ten kernels, with the channel count and outmode/inMode
+ * given to _init) on the HIP stream `hipStream` (a hipStream_t; NULL = the
+ * null stream) and returns once it is enqueued. pX, pW, pAux and pY are the
+ * buffers that kernel's _kernel takes (pAux: pShift for the int16 kernels,
+ * pSf for the real-input/real-output kernels, unused for C2C), in device
+ * memory with the alignment the FFTLIB_FUNCTION_GPU style requires; pX is
+ * not written. The caller keeps them alive until the stream reaches the
+ * work. Long complex transforms use library scratch owned per (device,
+ * stream), so calls on different streams may overlap. Errors in the call's
+ * parameters are reported before anything is enqueued, as _kernel reports
+ * them.
+ *
+ * The host pipeline runs calls whose buffers are host memory (pageable or
+ * pinned) through `depth` (1..8) slots, each with its own non-blocking HIP
+ * stream, pinned staging and device buffers (grow-only, freed by
+ * pipelineDestroy). FFTLIB_GPU_pipelineSubmit copies the call's inputs into
+ * pinned staging, enqueues the host-to-device copies, the transform and the
+ * device-to-host copy, and returns; when all slots are busy it first waits
+ * for the oldest call. A call's output is written to its pY when its slot is
+ * reused or by FFTLIB_GPU_pipelineWait, which waits for every submitted call
+ * (in submission order); so an input that is an earlier call's output needs
+ * a pipelineWait in between. The plan in pBlock is copied at submit time.
+ * One pipeline serves one thread at a time; pipelineDestroy drops outputs
+ * that were never waited for. */
+typedef struct FFTLIB_GPU_Pipeline_s FFTLIB_GPU_Pipeline;
+FFTLIB_STATUS FFTLIB_GPU_kernelAsync(void *pBlock, const void *pX, const void *pW, const void *pAux, void *pY,
+                                     void *hipStream);
+FFTLIB_STATUS FFTLIB_GPU_pipelineCreate(FFTLIB_GPU_Pipeline **pipeline, uint32_t depth);
+FFTLIB_STATUS FFTLIB_GPU_pipelineSubmit(FFTLIB_GPU_Pipeline *pipeline, void *pBlock, const void *pX, const void *pW,
+                                        const void *pAux, void *pY);
+FFTLIB_STATUS FFTLIB_GPU_pipelineWait(FFTLIB_GPU_Pipeline *pipeline);
+FFTLIB_STATUS FFTLIB_GPU_pipelineDestroy(FFTLIB_GPU_Pipeline *pipeline);
+
 /* Waits for all work the library enqueued on its stream. */
 FFTLIB_STATUS FFTLIB_GPU_synchronize(void);
 

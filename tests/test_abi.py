@@ -289,6 +289,43 @@ def test_kernel_refuses_bad_calls_before_gpu_work(lib, mem):
     assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_OPTIMIZED) == OK
 
 
+def test_async_entry_points_refuse_bad_calls_before_gpu_work(lib, mem):
+    """FFTLIB_GPU_kernelAsync and the host pipeline validate the plan and the
+    buffers before anything touches the GPU, with _kernel's status codes."""
+    X, W, Y, S = mem.p(0), mem.p(1), mem.p(2), mem.p(3)
+    f32 = F.FFTLIB_FLOAT32
+    name = "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc"
+    pb = ctypes.create_string_buffer(F.PBLOCK_SIZE[name])
+    d, dw = bp(f32, 2 * 64 * 2), bp(f32, 128)
+    run = lib.FFTLIB_GPU_kernelAsync
+    assert run(None, X, W, None, Y, None) == NULLP
+    assert run(pb, X, W, None, Y, None) == INVAL                       # never initialised
+    assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_GPU) == OK
+    assert getattr(lib, name + "_init")(X, d, W, dw, Y, d, 64, 2, pb) == OK
+    assert run(pb, None, W, None, Y, None) == NULLP
+    assert run(pb, X, W, None, Y + 4, None) == ALIGN                   # device buffers: 8-byte aligned
+    pbr = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_fft1dBatched_i32f_c32fc_o32fc"])
+    assert lib.FFTLIB_fft1dBatched_i32f_c32fc_o32fc_init(X, d, W, d, X, d, S, d, Y, d, 64, 2, 0, pbr) == OK
+    assert run(pbr, X, W, None, Y, None) == NULLP                      # real transforms need pSf
+    pbi = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_fft1dBatched_i16sc_c16sc_o16sc"])
+    i16, u32 = F.FFTLIB_INT16, F.FFTLIB_UINT32
+    assert lib.FFTLIB_fft1dBatched_i16sc_c16sc_o16sc_init(X, bp(i16, 512), W, bp(i16, 512), Y, bp(i16, 512), S,
+                                                           bp(u32, 2), 256, 1, pbi) == OK
+    assert run(pbi, X, W, S, Y, None) == DIM                           # 4 radix-4 stages, 2 shifts
+    assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_NATC) == OK
+    assert getattr(lib, name + "_init")(X, d, W, dw, Y, d, 64, 2, pb) == OK
+    assert run(pb, X, W, None, Y, None) == NOTIMPL
+    assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_OPTIMIZED) == OK
+    handle = ctypes.c_void_p(1)
+    assert lib.FFTLIB_GPU_pipelineCreate(ctypes.byref(handle), 0) == INVAL
+    assert handle.value is None
+    assert lib.FFTLIB_GPU_pipelineCreate(ctypes.byref(handle), 9) == INVAL
+    assert lib.FFTLIB_GPU_pipelineCreate(None, 2) == NULLP
+    assert lib.FFTLIB_GPU_pipelineSubmit(None, pb, X, W, None, Y) == NULLP
+    assert lib.FFTLIB_GPU_pipelineWait(None) == NULLP
+    assert lib.FFTLIB_GPU_pipelineDestroy(None) == NULLP
+
+
 # ---- table generators -----------------------------------------------------------
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 15, 16, 30, 60, 64, 120, 960, 1000, 1024, 3125, 4096, 6250, 15625,
                                16384])
