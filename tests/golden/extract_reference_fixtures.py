@@ -9,7 +9,11 @@ those two data files (C array literals and CSV, no code is executed), checks
 that every array has the length the listed shape implies, and keeps the cases
 small enough for the CPU test suite in tests/golden/ref_<alias>.npz.
 
-It runs only where the reference tree is present; the tests read the .npz.
+It also copies the ten kernels' test lists (data files of the reference's
+test generator) to tests/golden/testlists/<kernel>.csv, so that the lists run
+unchanged against the GPU route without the reference tree.
+
+It runs only where the reference tree is present; the tests read the copies.
 Usage: python tests/golden/extract_reference_fixtures.py [reference_root]
 """
 import csv
@@ -71,8 +75,28 @@ def parse_arrays(path):
     return arrays
 
 
+LISTED = ["FFTLIB_fft1dBatched_i32fc_c32fc_o32fc", "FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc",
+          "FFTLIB_fft1d_i32fc_c32fc_o32fc", "FFTLIB_ifft1d_i32fc_c32fc_o32fc",
+          "FFTLIB_fft1dBatched_i16sc_c16sc_o16sc", "FFTLIB_fft1d_i16sc_c16sc_o16sc",
+          "FFTLIB_fft1dBatched_i32f_c32fc_o32fc", "FFTLIB_fft1d_i32f_c32fc_o32fc",
+          "FFTLIB_ifft1dBatched_i32fc_c32fc_o32f", "FFTLIB_ifft1d_i32fc_c32fc_o32f"]
+
+
+def copy_test_lists(ref_root):
+    dst_dir = os.path.join(HERE, "testlists")
+    os.makedirs(dst_dir, exist_ok=True)
+    for kdir in LISTED:
+        src = os.path.join(ref_root, "test/fft_c7x/FFTLIB_idat_gen", kdir, "test_cases_list.csv")
+        with open(src, "rb") as fh:
+            data = fh.read()
+        with open(os.path.join(dst_dir, kdir + ".csv"), "wb") as fh:
+            fh.write(data)
+    print(f"copied {len(LISTED)} test lists -> {dst_dir}")
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    copy_test_lists(ref_root)
     for alias, (kdir, shape_of) in KERNELS.items():
         csv_path = os.path.join(ref_root, "test/fft_c7x/FFTLIB_idat_gen", kdir, "test_cases_list.csv")
         data_dir = os.path.join(ref_root, "test/fft_c7x", kdir, "test_data/extended_test_cases")
