@@ -22,7 +22,10 @@ r2c_4096 (R2C then C2R, N=4096, 16384 channels), ragged_10000 (10000 C2C
 transforms of lengths drawn log-uniformly from [64, 65536] and snapped to
 2/3/5-smooth lengths, forward then inverse in one graph launch),
 copy_503m (plain device copy of the N=960 workload's bytes, for the HBM
-counter calibration).
+counter calibration), c2c_960_host (the N=960 workload with input and output
+in pinned host memory, streamed through the library's host pipeline with two
+slots: the PCIe-inclusive rate; the synchronous host-pointer style is timed
+beside it).
 """
 import argparse
 import ctypes
@@ -46,7 +49,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "ragged_10000", "copy_503m"])
+    ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "ragged_10000", "copy_503m",
+                             "c2c_960_host"])
     ap.add_argument("--no-tune", action="store_true", help="ragged_10000: skip the autotuning pass")
     ap.add_argument("--variant", default="fp32", choices=["fp32", "bf16x3", "fp16x2", "bf16x2"],
                     help="C2C arithmetic: fp32 mixed radix, or MFMA DFT-as-GEMM with split bf16/fp16 operands")
@@ -165,6 +169,77 @@ class Workload:
             res["all_cores"] = {"value": flops / float(np.median(tt)) / 1e9, "cores": threads,
                                 "sample": "same sample, channels split over POSIX threads"}
         return res
+
+    # -- the N=960 workload with host-resident buffers (PCIe-inclusive) --------
+    def _setup_c2c_960_host(self):
+        F, lib, torch = self.F, self.lib, self.torch
+        self._setup_c2c_960()
+        n, b = self.n, self.b
+        # pinned host copies of the input and output; the device buffers of
+        # the device workload are released
+        self.hx = torch.from_numpy(self.host_x.view(np.float32).reshape(-1)).pin_memory()
+        self.hw = torch.from_numpy(self.host_w).pin_memory()
+        self.hy = torch.empty(2 * n * b, dtype=torch.float32).pin_memory()
+        self.x = self.w = None
+        self.y = None
+        torch.cuda.empty_cache()
+        self.pipe = ctypes.c_void_p()
+        assert lib.FFTLIB_GPU_pipelineCreate(ctypes.byref(self.pipe), 2) == 0, F.last_error()
+        self.kernel_name = "fixed_kernel<960, C2C>"
+        self.wall_timing = True
+        self.config = dict(self.config, workload="FFTLIB_fft1dBatched_i32fc_c32fc_o32fc N=960 B=65536 (fp32 C2C), "
+                                                 "input and output in pinned host memory, host pipeline depth 2",
+                           pcie="PCIe-inclusive: 8 B per point each way")
+        xp, wp, yp, pb = self.hx.data_ptr(), self.hw.data_ptr(), self.hy.data_ptr(), self.pblock
+
+        def step():
+            st = lib.FFTLIB_GPU_pipelineSubmit(self.pipe, pb, xp, wp, None, yp)
+            if st != 0:
+                raise RuntimeError(f"pipeline status {st}: {F.last_error()}")
+        self.step = step
+
+        def finish():
+            st = lib.FFTLIB_GPU_pipelineWait(self.pipe)
+            if st != 0:
+                raise RuntimeError(f"pipeline wait status {st}: {F.last_error()}")
+        self.finish = finish
+
+    def extra_c2c_960_host(self, steps):
+        """The same transform through the synchronous host-pointer style
+        (FFTLIB_FUNCTION_OPTIMIZED: copy in, transform, copy out, wait)."""
+        F, lib = self.F, self.lib
+        n, b = self.n, self.b
+        pb = ctypes.create_string_buffer(F.PBLOCK_SIZE["FFTLIB_fft1dBatched_i32fc_c32fc_o32fc"])
+        assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_OPTIMIZED) == 0
+        bx, bw = ctypes.byref(self.bp_x), ctypes.byref(self.bp_w)
+        xp, wp, yp = self.hx.data_ptr(), self.hw.data_ptr(), self.hy.data_ptr()
+        assert lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_init(xp, bx, wp, bw, yp, bx, n, b, pb) == 0
+        fn = lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_kernel
+        assert fn(xp, bx, wp, bw, yp, bx, n, b, pb) == 0, F.last_error()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert fn(xp, bx, wp, bw, yp, bx, n, b, pb) == 0, F.last_error()
+        t = (time.perf_counter() - t0) / steps
+        assert lib.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_GPU) == 0
+        return {"host_sync_style": {"ms_per_step": round(t * 1e3, 3),
+                                    "gflops": round(self.flops / t / 1e9, 1),
+                                    "host_gb_s_each_way": round(8.0 * n * b / t / 1e9, 2)}}
+
+    def check_c2c_960_host(self):
+        """Relative L2 of 256 channels of the pinned host output against the
+        FP64 DFT and the oracle."""
+        import oracle
+        idx = np.arange(0, self.b, self.b // 256)
+        y = self.hy.numpy().reshape(self.b, 2 * self.n)[idx].copy().view(np.complex64)
+        x = self.host_x[idx]
+        ref64 = np.fft.fft(x.astype(np.complex128), axis=1)
+        refo = oracle.c2c(x, self.n, len(idx))
+        return {"rel_l2_vs_fp64": float(np.linalg.norm(y - ref64) / np.linalg.norm(ref64)),
+                "rel_l2_vs_oracle": float(np.linalg.norm(y - refo) / np.linalg.norm(refo)),
+                "checked_channels": int(len(idx))}
+
+    def cpu_c2c_960_host(self, seconds):
+        return self.cpu_c2c_960(seconds)
 
     # -- plain device copy of the C2C workload's input size (counter calibration)
     # -- ragged batch: 10000 C2C transforms, forward then inverse -----------
@@ -470,11 +545,15 @@ def main():
     ev0.record(stream)
     for _ in range(args.steps):
         wl.step()
+    if hasattr(wl, "finish"):
+        wl.finish()
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     lib.FFTLIB_GPU_setSynchronous(1)
     ms = ev0.elapsed_time(ev1) / args.steps
+    if getattr(wl, "wall_timing", False):
+        ms = wall * 1e3 / args.steps       # host-to-host: the pipeline's own streams
     t = ms / 1e3
     achieved = wl.bytes / t / 1e9
     value = wl.flops / t / 1e9 if wl.flops else achieved
@@ -491,6 +570,14 @@ def main():
                      "timing": "HIP events on the launch stream, average over the timed steps"},
         "wall_s_timed": round(wall, 4),
     }
+    if getattr(wl, "wall_timing", False):
+        out["roofline"] = {"bound": "pcie", "achieved": round(wl.bytes / 2 / t / 1e9, 2), "peak": 64.0,
+                           "unit": "GB/s", "frac": round(wl.bytes / 2 / t / 1e9 / 64.0, 4), "traffic": None,
+                           "kernel": wl.kernel_name,
+                           "timing": "wall clock over the timed submits and the final pipeline wait; bytes "
+                                     "per direction against PCIe Gen5 x16 (64 GB/s each way)"}
+    if hasattr(wl, "extra_" + args.workload):
+        out.update(getattr(wl, "extra_" + args.workload)(max(3, args.steps // 4)))
     out["check"] = getattr(wl, "check_" + args.workload)()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = getattr(wl, "cpu_" + args.workload)(args.cpu_seconds)

@@ -3,14 +3,16 @@
 // FFTLIB_GPU_kernelAsync enqueues the transform an _init call planned in a
 // pBlock on the caller's HIP stream, with device buffers, and returns.
 //
-// The host pipeline serves callers whose data lives in (pageable) host
-// memory. Each of its `depth` slots owns a HIP stream, pinned staging for the
-// call's inputs and output, and device buffers. A submit copies the inputs
-// into the slot's pinned staging (CPU), then enqueues host-to-device copies,
+// The host pipeline serves callers whose data lives in host memory. Each of
+// its `depth` slots owns a HIP stream, pinned staging for the call's inputs
+// and output, and device buffers. A submit enqueues host-to-device copies,
 // the transform and the device-to-host copy on the slot's stream and
-// returns; the output reaches the caller's pY when the slot is next reused
-// or at pipelineWait. With two or more slots the PCIe copies of one call
-// overlap the transform of the previous one.
+// returns. Buffers the runtime can reach directly (pinned host memory from
+// hipHostMalloc, or device memory) are copied by DMA straight from and to
+// the caller's pointers; pageable inputs are first copied into the slot's
+// pinned staging on the CPU, and a pageable output reaches the caller's pY
+// when the slot is next reused or at pipelineWait. With two or more slots
+// the PCIe copies of one call overlap the transform of the previous one.
 #include <string.h>
 
 #include <vector>
@@ -39,6 +41,19 @@ FFTLIB_STATUS load_plan(const void *pBlock, Plan *p)
 }
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1u) / a * a; }
+
+// True for memory the GPU's copy engines can address directly: pinned host
+// memory or device memory. Pageable memory reports no HIP allocation (an
+// error status, cleared here so that it is not taken for a launch error).
+bool dma_reachable(const void *p)
+{
+   hipPointerAttribute_t attr;
+   if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+      (void) hipGetLastError();
+      return false;
+   }
+   return attr.type == hipMemoryTypeHost || attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
 
 struct Slot {
    hipStream_t stream = nullptr;
@@ -80,7 +95,7 @@ hipError_t finish(Slot &s)
    hipError_t e = hipEventSynchronize(s.done);
    s.busy       = false;
    if (e != hipSuccess) return e;
-   memcpy(s.pendingY, s.hostOut, s.pendingBytes);
+   if (s.pendingY) memcpy(s.pendingY, s.hostOut, s.pendingBytes);
    return hipSuccess;
 }
 
@@ -150,32 +165,51 @@ FFTLIB_STATUS FFTLIB_GPU_pipelineSubmit(FFTLIB_GPU_Pipeline *pipeline, void *pBl
    Slot &s = pipeline->slots[pipeline->next];
    if ((e = finish(s)) != hipSuccess) return fail("pipeline wait", e);
 
-   // staging layout: input | twiddles | aux, each 256-byte aligned
-   const size_t offTw  = round_up(b.in.bytes, 256);
-   const size_t offAux = offTw + round_up(b.tw.bytes, 256);
-   const size_t inAll  = offAux + round_up(b.aux0.bytes, 256);
-   auto hostAlloc = [](void **ptr, size_t n) { return hipHostMalloc(ptr, n, hipHostMallocDefault); };
-   auto hostFree  = [](void *ptr) { return hipHostFree(ptr); };
-   auto devAlloc  = [](void **ptr, size_t n) { return hipMalloc(ptr, n); };
-   auto devFree   = [](void *ptr) { return hipFree(ptr); };
-   if ((e = grow(&s.hostIn, &s.hostInCap, inAll, hostAlloc, hostFree)) != hipSuccess ||
-       (e = grow(&s.hostOut, &s.hostOutCap, b.outBytes, hostAlloc, hostFree)) != hipSuccess ||
+   // staging layout: input | twiddles | aux, each 256-byte aligned; pinned
+   // staging is only needed for buffers the copy engines cannot reach
+   const IoBuffer ins[3] = {b.in, b.tw, b.aux0};
+   const size_t   off[3] = {0, round_up(b.in.bytes, 256), round_up(b.in.bytes, 256) + round_up(b.tw.bytes, 256)};
+   const size_t   inAll  = off[2] + round_up(b.aux0.bytes, 256);
+   bool           reach[3];
+   bool           staged = false;
+   for (int i = 0; i < 3; ++i) {
+      reach[i] = ins[i].ptr == nullptr || dma_reachable(ins[i].ptr);
+      staged   = staged || !reach[i];
+   }
+   const bool direct    = dma_reachable(pY);
+   auto       hostAlloc = [](void **ptr, size_t n) { return hipHostMalloc(ptr, n, hipHostMallocDefault); };
+   auto       hostFree  = [](void *ptr) { return hipHostFree(ptr); };
+   auto       devAlloc  = [](void **ptr, size_t n) { return hipMalloc(ptr, n); };
+   auto       devFree   = [](void *ptr) { return hipFree(ptr); };
+   if ((staged && (e = grow(&s.hostIn, &s.hostInCap, inAll, hostAlloc, hostFree)) != hipSuccess) ||
+       (!direct && (e = grow(&s.hostOut, &s.hostOutCap, b.outBytes, hostAlloc, hostFree)) != hipSuccess) ||
        (e = grow(&s.devIn, &s.devInCap, inAll, devAlloc, devFree)) != hipSuccess ||
        (e = grow(&s.devOut, &s.devOutCap, b.outBytes, devAlloc, devFree)) != hipSuccess)
       return fail("pipeline staging allocation", e);
 
-   memcpy(s.hostIn, b.in.ptr, b.in.bytes);
-   memcpy(s.hostIn + offTw, b.tw.ptr, b.tw.bytes);
-   if (b.aux0.ptr) memcpy(s.hostIn + offAux, b.aux0.ptr, b.aux0.bytes);
-   if ((e = hipMemcpyAsync(s.devIn, s.hostIn, inAll, hipMemcpyHostToDevice, s.stream)) != hipSuccess)
-      return fail("hipMemcpyAsync (to device)", e);
-   const DeviceBuffers d{s.devIn, s.devIn + offTw, b.aux0.ptr ? s.devIn + offAux : nullptr, s.devOut};
+   for (int i = 0; i < 3; ++i) {
+      if (ins[i].ptr == nullptr || ins[i].bytes == 0) continue;
+      const void *src = ins[i].ptr;
+      if (!reach[i]) {
+         memcpy(s.hostIn + off[i], src, ins[i].bytes);
+         src = s.hostIn + off[i];
+      }
+      if ((e = hipMemcpyAsync(s.devIn + off[i], src, ins[i].bytes, hipMemcpyDefault, s.stream)) != hipSuccess)
+         return fail("hipMemcpyAsync (to device)", e);
+   }
+   const DeviceBuffers d{s.devIn, s.devIn + off[1], b.aux0.ptr ? s.devIn + off[2] : nullptr, s.devOut};
    if ((e = launch(p, d, s.stream)) != hipSuccess) return fail("kernel launch", e);
-   if ((e = hipMemcpyAsync(s.hostOut, s.devOut, b.outBytes, hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+   // outputs leave in submission order (two calls may write one pY)
+   const uint32_t n    = (uint32_t) pipeline->slots.size();
+   const Slot    &prev = pipeline->slots[(pipeline->next + n - 1u) % n];
+   if (n > 1u && prev.busy && (e = hipStreamWaitEvent(s.stream, prev.done, 0)) != hipSuccess)
+      return fail("hipStreamWaitEvent", e);
+   if ((e = hipMemcpyAsync(direct ? pY : (void *) s.hostOut, s.devOut, b.outBytes, hipMemcpyDefault, s.stream)) !=
+       hipSuccess)
       return fail("hipMemcpyAsync (to host)", e);
    if ((e = hipEventRecord(s.done, s.stream)) != hipSuccess) return fail("hipEventRecord", e);
    s.busy          = true;
-   s.pendingY      = pY;
+   s.pendingY      = direct ? nullptr : pY;
    s.pendingBytes  = b.outBytes;
    pipeline->next  = (pipeline->next + 1u) % (uint32_t) pipeline->slots.size();
    return FFTLIB_SUCCESS;

@@ -117,18 +117,21 @@ FFTLIB_STATUS FFTLIB_GPU_tuneLoad(const char *path);
  * parameters are reported before anything is enqueued, as _kernel reports
  * them.
  *
- * The host pipeline runs calls whose buffers are host memory (pageable or
- * pinned) through `depth` (1..8) slots, each with its own non-blocking HIP
- * stream, pinned staging and device buffers (grow-only, freed by
- * pipelineDestroy). FFTLIB_GPU_pipelineSubmit copies the call's inputs into
- * pinned staging, enqueues the host-to-device copies, the transform and the
- * device-to-host copy, and returns; when all slots are busy it first waits
- * for the oldest call. A call's output is written to its pY when its slot is
- * reused or by FFTLIB_GPU_pipelineWait, which waits for every submitted call
- * (in submission order); so an input that is an earlier call's output needs
- * a pipelineWait in between. The plan in pBlock is copied at submit time.
- * One pipeline serves one thread at a time; pipelineDestroy drops outputs
- * that were never waited for. */
+ * The host pipeline runs calls whose buffers are host memory through
+ * `depth` (1..8) slots, each with its own non-blocking HIP stream and device
+ * buffers (grow-only, freed by pipelineDestroy). FFTLIB_GPU_pipelineSubmit
+ * enqueues the host-to-device copies, the transform and the device-to-host
+ * copy, and returns; when all slots are busy it first waits for the oldest
+ * call. Pinned buffers (hipHostMalloc) and device buffers are copied by DMA
+ * from and to the caller's memory while the call runs: keep such inputs
+ * unchanged until the call has completed. Pageable inputs are copied into
+ * the slot's pinned staging before submit returns; a pageable output is
+ * written to its pY when the slot is reused or by FFTLIB_GPU_pipelineWait.
+ * pipelineWait waits for every submitted call (in submission order), so an
+ * input that is an earlier call's output needs a pipelineWait in between.
+ * The plan in pBlock is copied at submit time. One pipeline serves one
+ * thread at a time; pipelineDestroy drops outputs that were never waited
+ * for. */
 typedef struct FFTLIB_GPU_Pipeline_s FFTLIB_GPU_Pipeline;
 FFTLIB_STATUS FFTLIB_GPU_kernelAsync(void *pBlock, const void *pX, const void *pW, const void *pAux, void *pY,
                                      void *hipStream);

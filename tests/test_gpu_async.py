@@ -187,3 +187,72 @@ def test_host_pipeline_same_output_twice_keeps_the_later_call():
         assert L.FFTLIB_GPU_pipelineWait(pipe) == 0                      # nothing pending: no-op
     finally:
         assert L.FFTLIB_GPU_pipelineDestroy(pipe) == 0
+
+
+def _pinned(a):
+    """Copy of numpy array `a` in pinned host memory (torch pin_memory)."""
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a).view(np.int32) if a.dtype == np.uint32 else
+                         np.ascontiguousarray(a)).pin_memory()
+    return t, t.numpy()
+
+
+@pytest.mark.parametrize("kind,n,b,mode", [("c2c", 960, 64, 0), ("c2c", 1 << 20, 2, 0), ("i16", 256, 40, 0),
+                                           ("r2c", 4096, 17, 0), ("c2r", 62500, 2, 1)])
+def test_host_pipeline_pinned_buffers(kind, n, b, mode):
+    """Pinned inputs and outputs are copied by DMA straight from and to the
+    caller's memory; the results equal the synchronous ones."""
+    import torch
+    L = G.lib()
+    pipe = ctypes.c_void_p()
+    assert L.FFTLIB_GPU_pipelineCreate(ctypes.byref(pipe), 2) == 0
+    try:
+        keep, jobs = [], []
+        for i in range(3):
+            x, w, aux, (ycount, ydt), want, s = case(kind, n, b, mode, seed=300 + i)
+            assert s == 0
+            pb = plan(kind, n, b, mode, nshift=aux.size if kind == "i16" else 0)
+            tx, px = _pinned(x)
+            tw, pw = _pinned(w)
+            ta, pa = _pinned(aux) if aux is not None else (None, None)
+            ty = torch.full((ycount,), -7, dtype=torch.int16 if ydt == np.int16 else torch.float32).pin_memory()
+            keep.append((tx, tw, ta, ty, pb))
+            st = L.FFTLIB_GPU_pipelineSubmit(pipe, pb, tx.data_ptr(), tw.data_ptr(),
+                                             ta.data_ptr() if ta is not None else None, ty.data_ptr())
+            assert st == 0, G.F.last_error()
+            jobs.append((ty, want))
+        assert L.FFTLIB_GPU_pipelineWait(pipe) == 0
+        for ty, want in jobs:
+            y = ty.numpy()
+            v = np.uint16 if y.dtype == np.int16 else np.uint32
+            assert np.array_equal(y.view(v), np.ascontiguousarray(want).reshape(-1).view(y.dtype).view(v))
+    finally:
+        assert L.FFTLIB_GPU_pipelineDestroy(pipe) == 0
+
+
+def test_host_pipeline_pinned_same_output_keeps_the_later_call():
+    """Outputs copied by DMA on different slot streams still land in
+    submission order."""
+    import torch
+    L = G.lib()
+    pipe = ctypes.c_void_p()
+    assert L.FFTLIB_GPU_pipelineCreate(ctypes.byref(pipe), 3) == 0
+    try:
+        n, b = 1 << 20, 2                   # the long first call finishes last without ordering
+        xa, w, _, (ycount, _), _, _ = case("c2c", n, b, seed=7)
+        pb_long = plan("c2c", n, b)
+        ty = torch.zeros(ycount, dtype=torch.float32).pin_memory()
+        tx, _ = _pinned(xa)
+        tw, _ = _pinned(w)
+        assert L.FFTLIB_GPU_pipelineSubmit(pipe, pb_long, tx.data_ptr(), tw.data_ptr(), None, ty.data_ptr()) == 0
+        n2, b2 = 1024, 2048                 # same output bytes, much less work
+        xb, w2, _, (ycount2, _), want_b, _ = case("c2c", n2, b2, seed=8)
+        assert ycount2 == ycount
+        pb_short = plan("c2c", n2, b2)
+        tx2, _ = _pinned(xb)
+        tw2, _ = _pinned(w2)
+        assert L.FFTLIB_GPU_pipelineSubmit(pipe, pb_short, tx2.data_ptr(), tw2.data_ptr(), None, ty.data_ptr()) == 0
+        assert L.FFTLIB_GPU_pipelineWait(pipe) == 0
+        assert np.array_equal(ty.numpy().view(np.uint32), want_b.reshape(-1).view(np.complex64).view(np.uint32))
+    finally:
+        assert L.FFTLIB_GPU_pipelineDestroy(pipe) == 0
