@@ -115,16 +115,34 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    return run_complex(p.st, (const float *) d.in, (float *) d.out, (const float *) d.tw, p.numChannels, mode, s);
 }
 
-// Real transforms: one fused kernel (half-length complex FFT in LDS plus the
-// split) when the half length fits one workgroup; otherwise the complex FFT
-// and the split as two passes through library scratch.
+// Real transforms: the fused kernel of the half length (complex FFT in LDS
+// plus the split) where one is specialised; otherwise, from half length
+// kRealSplitMin on, the complex FFT on the multi-length / two-pass kernels
+// and the split as a second pass through library scratch; the generic fused
+// kernel for shorter half lengths, or when the specialised kernels are
+// switched off and the half length fits one workgroup. Measured on MI355X
+// (half lengths 20-17280, 1 to 2^25/N channels) the two-pass route is 2-15x
+// faster than the generic fused kernel, which runs one transform per
+// workgroup with its twiddles and index arithmetic unspecialised, except for
+// calls of at most kRealFusedMaxPoints half-length points, where the second
+// launch costs more (about 1 us) than it saves.
+constexpr uint32_t kRealSplitMin       = 16;
+constexpr uint64_t kRealFusedMaxPoints = 256;
+
+bool real_fused_generic(const Plan &p)
+{
+   if (!fits_lds(p.st)) return false;
+   return !specialized_kernels() || p.st.n < kRealSplitMin ||
+          (uint64_t) p.st.n * p.numChannels <= kRealFusedMaxPoints;
+}
+
 hipError_t run_r2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
    const float *x = (const float *) d.in, *tw = (const float *) d.tw, *sf = (const float *) d.aux0;
    float       *y = (float *) d.out;
    if (specialized_kernels() && fixed_real_supported(p.st.n))
       return launch_r2c_fixed(x, y, tw, sf, p.st.n, p.numChannels, p.mode, s);
-   if (fits_lds(p.st)) return launch_r2c(x, y, tw, sf, p.st, p.numChannels, p.mode, s);
+   if (real_fused_generic(p)) return launch_r2c(x, y, tw, sf, p.st, p.numChannels, p.mode, s);
    void      *z = nullptr;
    hipError_t e = device_scratch((size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
    if (e == hipSuccess) e = run_complex(p.st, x, (float *) z, tw, p.numChannels, C2C_FORWARD, s);
@@ -139,7 +157,7 @@ hipError_t run_c2r(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    const int    postdiv = p.kernel == K_C2R_SINGLE ? 1 : 0;
    if (specialized_kernels() && fixed_real_supported(p.st.n))
       return launch_c2r_fixed(x, y, tw, sf, p.st.n, p.numChannels, p.mode, postdiv, s);
-   if (fits_lds(p.st)) return launch_c2r(x, y, tw, sf, p.st, p.numChannels, p.mode, postdiv, s);
+   if (real_fused_generic(p)) return launch_c2r(x, y, tw, sf, p.st, p.numChannels, p.mode, postdiv, s);
    void      *z = nullptr;
    hipError_t e = device_scratch((size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
    if (e == hipSuccess) e = launch_c2r_presplit(x, (float *) z, sf, p.st.n, p.numChannels, p.mode, s);
