@@ -190,6 +190,7 @@ hipError_t launch_r2c_fixed(const float *x, float *y, const float *tw, const flo
    if (numChannels == 0) return hipSuccess;
    FixedArgs a{};
    a.x = (const float2 *) x, a.y = (float2 *) y, a.tw = (const float2 *) tw, a.sf = sf;
+   a.sf2 = ((uintptr_t) sf & 7u) == 0 ? (const float2 *) sf : nullptr;
    a.numChannels = numChannels, a.mode = C2C_FORWARD, a.aux = outmode, a.inStride = nc;
    a.outStride = outmode ? 2u * nc : nc + 1u;
    return dispatch_fixed_real<KIND_R2C>(nc, a, stream);
@@ -201,6 +202,7 @@ hipError_t launch_c2r_fixed(const float *x, float *y, const float *tw, const flo
    if (numChannels == 0) return hipSuccess;
    FixedArgs a{};
    a.x = (const float2 *) x, a.y = (float2 *) y, a.tw = (const float2 *) tw, a.sf = sf;
+   a.sf2 = ((uintptr_t) sf & 7u) == 0 ? (const float2 *) sf : nullptr;
    a.numChannels = numChannels, a.mode = postdiv ? C2C_INVERSE_POSTDIV : C2C_INVERSE_PRESCALE, a.aux = inMode;
    a.inStride = inMode ? 2u * nc : nc + 1u, a.outStride = nc;
    return dispatch_fixed_real<KIND_C2R>(nc, a, stream);

@@ -206,6 +206,7 @@ struct FixedArgs {
    float2       *y;
    const float2 *tw;
    const float  *sf;          // split factors (real transforms)
+   const float2 *sf2;         // the same table when 8-byte aligned (one load per factor), else null
    uint32_t      numChannels;
    uint32_t      numGroups;
    int           mode;        // C2CMode
@@ -242,6 +243,13 @@ __device__ __forceinline__ size_t out_base(const FixedArgs &a, uint32_t ch)
    return a.outOff ? (size_t) a.outOff[ch] : (size_t) ch * a.outStride;
 }
 
+// Complex entry j (floats 2j, 2j+1) of the split-factor table.
+__device__ __forceinline__ float2 split_factor(const FixedArgs &a, uint32_t j)
+{
+   if (a.sf2) return a.sf2[j];
+   return make_float2(a.sf[2u * j], a.sf[2u * j + 1u]);
+}
+
 // Point i (< N) of the complex sequence the FFT of channel `in` transforms:
 // the input itself (C2C, R2C), or the C2R pre-split
 // Z[i] = X[i] A'[i] + V[i] B'[i], V = conj(X[N-i]) (inMode 0) or X[N+i]
@@ -262,8 +270,8 @@ __device__ __forceinline__ float2 fixed_input(const float2 *in, uint32_t i, cons
          v1   = in[N - i];
          v1.y = -v1.y;
       }
-      const float2 A = make_float2(a.sf[2u * i], a.sf[2u * i + 1u]);
-      const float2 B = make_float2(a.sf[2u * N + 2u * i], a.sf[2u * N + 2u * i + 1u]);
+      const float2 A = split_factor(a, i);
+      const float2 B = split_factor(a, (uint32_t) N + i);
       return make_float2(v0.x * A.x - v0.y * A.y + (v1.x * B.x - v1.y * B.y),
                          v0.x * A.y + v0.y * A.x + (v1.x * B.y + v1.y * B.x));
    }
@@ -348,8 +356,12 @@ __device__ __forceinline__ float2 out_postdiv(float2 v, float n) { return make_f
 // LDS slots the store stage of work item t reads: outputs k = t + T*it of
 // its transform (and, for R2C, the mirrored N - k). They are the same for
 // every group of a persistent workgroup, so they are computed once when they
-// fit in a few registers (at most 16 outputs per work item).
-template <int N, int KIND, bool CACHE = true>
+// fit in a few registers (at most 16 outputs per work item). R2C computes
+// them per output instead: its store runs in chunks of 8 with the split
+// factors in flight, and indexing the cached arrays by the chunk would put
+// them in scratch memory, while unrolling every chunk needs more than 200
+// VGPRs.
+template <int N, int KIND, bool CACHE = (KIND != KIND_R2C)>
 struct StoreMap {
    static constexpr FixedPlan P      = Fixed<N>::P;
    static constexpr int       iters  = (N + P.T - 1) / P.T;
@@ -403,8 +415,10 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
    float2             *out   = a.y + out_base(a, ch);
    const float         sy    = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
    const bool          pdiv  = a.mode == C2C_INVERSE_POSTDIV;
-#pragma unroll 1
-   for (int c0 = 0; c0 < iters; c0 += CH) {
+   // one chunk of CH outputs starting at output index t + T*c0; with a cached
+   // map the chunk loop is unrolled so that the map's arrays are indexed by
+   // constants (a runtime index would put them in scratch memory)
+   auto chunk = [&](int c0) __attribute__((always_inline)) {
       float2 v[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) v[c] = src[map.at(c0 + c, t)];
@@ -417,8 +431,8 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
             const uint32_t i = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || i >= (uint32_t) N)) break;
             const float2 zi = v[c], zm = vm[c];
-            const float2 A  = make_float2(a.sf[2u * i], a.sf[2u * i + 1u]);
-            const float2 B  = make_float2(a.sf[4u * N - 2u * i - 2u], a.sf[4u * N - 2u * i - 1u]);
+            const float2 A  = split_factor(a, i);
+            const float2 B  = split_factor(a, 2u * N - 1u - i);
             float2       t1;
             if (i == 0u) t1 = make_float2(zi.x * B.x + zi.y * B.y, zi.x * B.x - zi.y * B.y);
             else t1 = make_float2(zm.x * B.x + zm.y * B.y, zm.x * B.y - zm.y * B.x);
@@ -444,6 +458,14 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
             out[k] = make_float2(v[c].x, sy * v[c].y);
          }
       }
+   };
+   if constexpr (StoreMap<N, KIND, C>::cached) {
+#pragma unroll
+      for (int c0 = 0; c0 < iters; c0 += CH) chunk(c0);
+   }
+   else {
+#pragma unroll 1
+      for (int c0 = 0; c0 < iters; c0 += CH) chunk(c0);
    }
 }
 
