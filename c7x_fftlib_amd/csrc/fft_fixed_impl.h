@@ -585,18 +585,59 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       }
    }
    else if constexpr (KIND == KIND_C2R && N % T == 0 && N >= 256) {
-      // C2R: work item t computes the pre-split points t + T*it of its
-      // transform from coalesced (and mirrored) reads, 8 at a time, into LDS;
-      // then every pass runs from LDS.
+      // C2R: the pre-split points go from coalesced HBM reads into LDS, then
+      // every pass runs from LDS. inMode 0: work item t takes the mirror
+      // pairs (i, N-i), i = t + T*it < N/2, reading each input bin once
+      // (X[i] and X[N-i] feed both Z[i] and Z[N-i]); i = 0 pairs bin 0 with
+      // the Nyquist bin N and also forms the self-mirrored Z[N/2]. inMode 1
+      // (V[i] = X[N+i], no mirror): work item t computes points t + T*it, 8
+      // at a time.
       if constexpr (fixed_tw_lds<N>()) {
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
       }
       constexpr int iters = N / T;
       constexpr int CH    = largest_divisor_upto(iters, 8);
+      constexpr int HALF  = iters / 2;   // pairs per work item (N/2 = T * HALF)
+      constexpr int PCH   = largest_divisor_upto(HALF > 0 ? HALF : 1, 4);
+      // Z = X A' + conj(Xm) B' for a bin X, its mirror Xm and factors A', B'
+      auto presplit = [](float2 x, float2 xm, float2 A, float2 B) __attribute__((always_inline)) {
+         return make_float2(x.x * A.x - x.y * A.y + (xm.x * B.x + xm.y * B.y),
+                            x.x * A.y + x.y * A.x + (xm.x * B.y - xm.y * B.x));
+      };
       for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
-         if (fl < nf) {
+         if (fl < nf && a.aux == 0u && HALF >= 1 && (N / 2) % T == 0) {
+            const float2 *in = a.x + in_base(a, first + fl);
+#pragma unroll 1
+            for (int c0 = 0; c0 < HALF; c0 += PCH) {
+               float2 xi[PCH], xm[PCH];
+#pragma unroll
+               for (int c = 0; c < PCH; ++c) {
+                  const uint32_t i = t + (uint32_t) ((c0 + c) * T);
+                  xi[c]            = in[i];
+                  xm[c]            = in[(uint32_t) N - i];   // i = 0: the Nyquist bin
+               }
+#pragma unroll
+               for (int c = 0; c < PCH; ++c) {
+                  const uint32_t i  = t + (uint32_t) ((c0 + c) * T);
+                  const float2   zi = presplit(xi[c], xm[c], split_factor(a, i), split_factor(a, (uint32_t) N + i));
+                  data[fslot(i)]    = make_float2(sc.sx * zi.x, sc.sy * zi.y);
+                  if (i != 0u) {
+                     const uint32_t j  = (uint32_t) N - i;
+                     const float2   zj = presplit(xm[c], xi[c], split_factor(a, j), split_factor(a, (uint32_t) N + j));
+                     data[fslot(j)]    = make_float2(sc.sx * zj.x, sc.sy * zj.y);
+                  }
+                  else {
+                     constexpr uint32_t H  = (uint32_t) N / 2u;
+                     const float2       xh = in[H];
+                     const float2       zh = presplit(xh, xh, split_factor(a, H), split_factor(a, (uint32_t) N + H));
+                     data[fslot(H)]        = make_float2(sc.sx * zh.x, sc.sy * zh.y);
+                  }
+               }
+            }
+         }
+         else if (fl < nf) {
             const float2 *in = a.x + in_base(a, first + fl);
 #pragma unroll 1
             for (int c0 = 0; c0 < iters; c0 += CH) {
