@@ -597,8 +597,8 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       }
       constexpr int iters = N / T;
       constexpr int CH    = largest_divisor_upto(iters, 8);
-      constexpr int HALF  = iters / 2;   // pairs per work item (N/2 = T * HALF)
-      constexpr int PCH   = largest_divisor_upto(HALF > 0 ? HALF : 1, 4);
+      constexpr int HALF  = ((N / 2) + T - 1) / T;   // pairs per work item
+      constexpr int PCH   = largest_divisor_upto(HALF, 4);
       // Z = X A' + conj(Xm) B' for a bin X, its mirror Xm and factors A', B'
       auto presplit = [](float2 x, float2 xm, float2 A, float2 B) __attribute__((always_inline)) {
          return make_float2(x.x * A.x - x.y * A.y + (xm.x * B.x + xm.y * B.y),
@@ -607,20 +607,23 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
-         if (fl < nf && a.aux == 0u && HALF >= 1 && (N / 2) % T == 0) {
+         if (fl < nf && a.aux == 0u) {
             const float2 *in = a.x + in_base(a, first + fl);
 #pragma unroll 1
             for (int c0 = 0; c0 < HALF; c0 += PCH) {
                float2 xi[PCH], xm[PCH];
 #pragma unroll
                for (int c = 0; c < PCH; ++c) {
-                  const uint32_t i = t + (uint32_t) ((c0 + c) * T);
+                  // pairs past N/2 (when T does not divide N/2) re-read pair
+                  // N/2 - 1 and store nothing
+                  const uint32_t i = min(t + (uint32_t) ((c0 + c) * T), (uint32_t) N / 2u - 1u);
                   xi[c]            = in[i];
                   xm[c]            = in[(uint32_t) N - i];   // i = 0: the Nyquist bin
                }
 #pragma unroll
                for (int c = 0; c < PCH; ++c) {
                   const uint32_t i  = t + (uint32_t) ((c0 + c) * T);
+                  if ((N / 2) % T != 0 && i >= (uint32_t) N / 2u) break;
                   const float2   zi = presplit(xi[c], xm[c], split_factor(a, i), split_factor(a, (uint32_t) N + i));
                   data[fslot(i)]    = make_float2(sc.sx * zi.x, sc.sy * zi.y);
                   if (i != 0u) {
