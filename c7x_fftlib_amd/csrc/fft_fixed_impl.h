@@ -243,6 +243,20 @@ __device__ __forceinline__ size_t out_base(const FixedArgs &a, uint32_t ch)
    return a.outOff ? (size_t) a.outOff[ch] : (size_t) ch * a.outStride;
 }
 
+// Streaming (non-temporal) global accesses: transform inputs are read once
+// and outputs written once, so they bypass the caches' retention policy.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 load_stream(const float2 *p)
+{
+   const f32x2_t t = __builtin_nontemporal_load((const f32x2_t *) p);
+   return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void store_stream(float2 *p, float2 v)
+{
+   const f32x2_t t = {v.x, v.y};
+   __builtin_nontemporal_store(t, (f32x2_t *) p);
+}
+
 // Complex entry j (floats 2j, 2j+1) of the split-factor table.
 __device__ __forceinline__ float2 split_factor(const FixedArgs &a, uint32_t j)
 {
@@ -258,7 +272,7 @@ template <int N, int KIND>
 __device__ __forceinline__ float2 fixed_input(const float2 *in, uint32_t i, const FixedArgs &a)
 {
    if constexpr (KIND != KIND_C2R) {
-      return in[i];
+      return load_stream(in + i);
    }
    else {
       const float2 v0 = in[i];
@@ -447,7 +461,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k] = out_postdiv(v[c], (float) N);
+            store_stream(out + k, out_postdiv(v[c], (float) N));
          }
       }
       else {
@@ -455,7 +469,7 @@ __device__ __forceinline__ void fixed_store(const float2 *src, uint32_t ch, uint
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = t + (uint32_t) ((c0 + c) * T);
             if (!EXACT && (c0 + c >= iters || k >= (uint32_t) N)) break;
-            out[k] = make_float2(v[c].x, sy * v[c].y);
+            store_stream(out + k, make_float2(v[c].x, sy * v[c].y));
          }
       }
    };
