@@ -11,6 +11,20 @@
 namespace fftlib {
 namespace engine {
 
+// Streaming (non-temporal) global accesses: transform inputs are read once
+// and outputs written once, so they bypass the caches' retention policy.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 load_stream(const float2 *p)
+{
+   const f32x2_t t = __builtin_nontemporal_load((const f32x2_t *) p);
+   return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void store_stream(float2 *p, float2 v)
+{
+   const f32x2_t t = {v.x, v.y};
+   __builtin_nontemporal_store(t, (f32x2_t *) p);
+}
+
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------

@@ -243,20 +243,6 @@ __device__ __forceinline__ size_t out_base(const FixedArgs &a, uint32_t ch)
    return a.outOff ? (size_t) a.outOff[ch] : (size_t) ch * a.outStride;
 }
 
-// Streaming (non-temporal) global accesses: transform inputs are read once
-// and outputs written once, so they bypass the caches' retention policy.
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 load_stream(const float2 *p)
-{
-   const f32x2_t t = __builtin_nontemporal_load((const f32x2_t *) p);
-   return make_float2(t.x, t.y);
-}
-__device__ __forceinline__ void store_stream(float2 *p, float2 v)
-{
-   const f32x2_t t = {v.x, v.y};
-   __builtin_nontemporal_store(t, (f32x2_t *) p);
-}
-
 // Complex entry j (floats 2j, 2j+1) of the split-factor table.
 __device__ __forceinline__ float2 split_factor(const FixedArgs &a, uint32_t j)
 {

@@ -40,6 +40,7 @@
 #include <mutex>
 #include <stdint.h>
 
+#include "fft_engine.h"
 #include "fft_kernels.h"
 
 namespace fftlib {
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
    float2         xin[15];
    if (f < a.numChannels) {
 #pragma unroll
-      for (int j = 0; j < 15; ++j) xin[j] = a.x[(size_t) f * kN + lane + 64 * j];
+      for (int j = 0; j < 15; ++j) xin[j] = engine::load_stream(a.x + (size_t) f * kN + lane + 64 * j);
    }
    for (; f < a.numChannels; f += stride) {
       // ---- stage the input (x[64a + b] at row a, column b) --------------------
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
       const uint32_t fn = f + stride;
       if (fn < a.numChannels) {
 #pragma unroll
-         for (int j = 0; j < 15; ++j) xin[j] = a.x[(size_t) fn * kN + lane + 64 * j];
+         for (int j = 0; j < 15; ++j) xin[j] = engine::load_stream(a.x + (size_t) fn * kN + lane + 64 * j);
       }
       // ---- GEMM 1: C1 (32 x 64) = A1 (32 x 32) * B1 (32 x 64) -------------------
       // B1 row k: k < 15 -> Re x[a = k][b], 16 <= k < 31 -> Im x[a = k - 16][b].
@@ -298,7 +299,7 @@ __global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
       }
       float2 *yo = a.y + (size_t) f * kN;
 #pragma unroll
-      for (int j = 0; j < 15; ++j) yo[lane + 64 * j] = rc[lane + 64 * j];
+      for (int j = 0; j < 15; ++j) engine::store_stream(yo + lane + 64 * j, rc[lane + 64 * j]);
    }
 }
 
