@@ -156,7 +156,7 @@ __device__ __forceinline__ float wave_max(float v)
 }
 
 template <int S>
-__global__ void __launch_bounds__(kThreadsM) c2c960_mfma_kernel(MfmaArgs a)
+__global__ void __launch_bounds__(kThreadsM) __attribute__((amdgpu_waves_per_eu(3, 3))) c2c960_mfma_kernel(MfmaArgs a)
 {
    constexpr int P = SplitTraits<S>::parts;
    extern __shared__ float smf[];
