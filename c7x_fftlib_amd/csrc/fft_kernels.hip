@@ -213,6 +213,52 @@ __device__ __forceinline__ int shift_round(int v, uint32_t s)
    return wrap16((v & mask) != 0 ? sh + 1 : sh);
 }
 
+// One radix-4 DIF butterfly of the int16 kernels ($I16B/_cn.c butterfly /
+// butterflyNoTw): on the packed points at offsets 0, q, 2q, 3q of its block,
+// 16-bit wrapping sums, the saturating rotation, Q15 twiddle products
+// w[p], w[q+p], w[2q+p] unless noTw, the rounding shift unless the stage is
+// the 8-point one (whose shift the final radix-2 stage applies), and the
+// outputs in the reference's storage order (o0, o2, o1, o3).
+__device__ __forceinline__ void i16_bfly4(uint32_t &v0, uint32_t &v1, uint32_t &v2, uint32_t &v3, const uint32_t *w,
+                                          uint32_t p, uint32_t q, bool noTw, bool doShift, uint32_t sh)
+{
+   const int s1r = wrap16(lo16(v0) + lo16(v2)), s1i = wrap16(hi16(v0) + hi16(v2));
+   const int s2r = wrap16(lo16(v1) + lo16(v3)), s2i = wrap16(hi16(v1) + hi16(v3));
+   const int d1r = wrap16(lo16(v0) - lo16(v2)), d1i = wrap16(hi16(v0) - hi16(v2));
+   const int d2r = wrap16(lo16(v1) - lo16(v3)), d2i = wrap16(hi16(v1) - hi16(v3));
+   int       o0r = wrap16(s1r + s2r), o0i = wrap16(s1i + s2i);
+   int       o1r = wrap16(d1r - rot90_re(d2i)), o1i = wrap16(d1i - d2r);
+   int       o2r = wrap16(s1r - s2r), o2i = wrap16(s1i - s2i);
+   int       o3r = wrap16(d1r + rot90_re(d2i)), o3i = wrap16(d1i + d2r);
+   if (!noTw) {
+      const uint32_t w1 = w[p], w2 = w[q + p], w3 = w[2 * q + p];
+      int            t;
+      t   = q15_round_sat(o1r * lo16(w1) - o1i * hi16(w1));
+      o1i = q15_round_sat(o1r * hi16(w1) + o1i * lo16(w1));
+      o1r = t;
+      t   = q15_round_sat(o2r * lo16(w2) - o2i * hi16(w2));
+      o2i = q15_round_sat(o2r * hi16(w2) + o2i * lo16(w2));
+      o2r = t;
+      t   = q15_round_sat(o3r * lo16(w3) - o3i * hi16(w3));
+      o3i = q15_round_sat(o3r * hi16(w3) + o3i * lo16(w3));
+      o3r = t;
+   }
+   if (doShift) {
+      o0r = shift_round(o0r, sh);
+      o0i = shift_round(o0i, sh);
+      o1r = shift_round(o1r, sh);
+      o1i = shift_round(o1i, sh);
+      o2r = shift_round(o2r, sh);
+      o2i = shift_round(o2i, sh);
+      o3r = shift_round(o3r, sh);
+      o3i = shift_round(o3i, sh);
+   }
+   v0 = pack16(o0r, o0i);
+   v1 = pack16(o2r, o2i);
+   v2 = pack16(o1r, o1i);
+   v3 = pack16(o3r, o3i);
+}
+
 __global__ void __launch_bounds__(kThreads) i16_lds_kernel(const uint32_t *__restrict__ x, uint32_t *__restrict__ y,
                                                            const uint32_t *__restrict__ tw,
                                                            const uint32_t *__restrict__ shift, StageTable st,
@@ -244,43 +290,12 @@ __global__ void __launch_bounds__(kThreads) i16_lds_kernel(const uint32_t *__res
          const uint32_t dft = k >> lq;
          const uint32_t p   = k & (q - 1u);
          uint32_t      *b0  = data + f * N + dft * M + p;
-         const uint32_t v0 = b0[0], v1 = b0[q], v2 = b0[2 * q], v3 = b0[3 * q];
-         const int s1r = wrap16(lo16(v0) + lo16(v2)), s1i = wrap16(hi16(v0) + hi16(v2));
-         const int s2r = wrap16(lo16(v1) + lo16(v3)), s2i = wrap16(hi16(v1) + hi16(v3));
-         const int d1r = wrap16(lo16(v0) - lo16(v2)), d1i = wrap16(hi16(v0) - hi16(v2));
-         const int d2r = wrap16(lo16(v1) - lo16(v3)), d2i = wrap16(hi16(v1) - hi16(v3));
-         int       o0r = wrap16(s1r + s2r), o0i = wrap16(s1i + s2i);
-         int       o1r = wrap16(d1r - rot90_re(d2i)), o1i = wrap16(d1i - d2r);
-         int       o2r = wrap16(s1r - s2r), o2i = wrap16(s1i - s2i);
-         int       o3r = wrap16(d1r + rot90_re(d2i)), o3i = wrap16(d1i + d2r);
-         const bool noTw = (M == 4u) || (M == 8u && p == 0u);
-         if (!noTw) {
-            const uint32_t w1 = w[p], w2 = w[q + p], w3 = w[2 * q + p];
-            int            t;
-            t   = q15_round_sat(o1r * lo16(w1) - o1i * hi16(w1));
-            o1i = q15_round_sat(o1r * hi16(w1) + o1i * lo16(w1));
-            o1r = t;
-            t   = q15_round_sat(o2r * lo16(w2) - o2i * hi16(w2));
-            o2i = q15_round_sat(o2r * hi16(w2) + o2i * lo16(w2));
-            o2r = t;
-            t   = q15_round_sat(o3r * lo16(w3) - o3i * hi16(w3));
-            o3i = q15_round_sat(o3r * hi16(w3) + o3i * lo16(w3));
-            o3r = t;
-         }
-         if (M != 8u) {
-            o0r = shift_round(o0r, sh);
-            o0i = shift_round(o0i, sh);
-            o1r = shift_round(o1r, sh);
-            o1i = shift_round(o1i, sh);
-            o2r = shift_round(o2r, sh);
-            o2i = shift_round(o2i, sh);
-            o3r = shift_round(o3r, sh);
-            o3i = shift_round(o3i, sh);
-         }
-         b0[0]     = pack16(o0r, o0i);
-         b0[q]     = pack16(o2r, o2i);
-         b0[2 * q] = pack16(o1r, o1i);
-         b0[3 * q] = pack16(o3r, o3i);
+         uint32_t       v0 = b0[0], v1 = b0[q], v2 = b0[2 * q], v3 = b0[3 * q];
+         i16_bfly4(v0, v1, v2, v3, w, p, q, (M == 4u) || (M == 8u && p == 0u), M != 8u, sh);
+         b0[0]     = v0;
+         b0[q]     = v1;
+         b0[2 * q] = v2;
+         b0[3 * q] = v3;
       }
       __syncthreads();
       twOff += 3u * q;
@@ -303,6 +318,126 @@ __global__ void __launch_bounds__(kThreads) i16_lds_kernel(const uint32_t *__res
       const uint32_t f = i / N;
       const uint32_t k = i - f * N;
       y[base + i]      = data[f * N + dif_position(k, st)];
+   }
+}
+
+// int16 FFT of N = 4^K points (16 <= N <= 4096), specialised per length:
+// T = N/16 work items per transform and 256/T transforms per workgroup; the
+// rows are staged into LDS with 16-byte coalesced loads (one pad word per 16
+// points), the radix-4 stages run two at a time with 16 points per work item
+// in registers (the last stage alone when K is odd), and the bit-reversed
+// result leaves with 16-byte stores. The butterfly arithmetic is
+// i16_bfly4's, as in i16_lds_kernel, so the results are identical.
+constexpr int i16_log4(int n) { return n <= 1 ? 0 : 1 + i16_log4(n / 4); }
+
+__device__ __forceinline__ uint32_t i16_slot(uint32_t i) { return i + (i >> 4); }
+
+// twiddle offset of radix-4 stage t (stage t has q = N / 4^(t+1) and uses 3q entries)
+template <int N>
+constexpr int i16_tw_off(int t)
+{
+   int off = 0, q = N / 4;
+   for (int r = 0; r < t; ++r, q /= 4) off += 3 * q;
+   return off;
+}
+
+// Stages S and S+1 (block sizes M = N / 4^S and M/4) on the 16 points
+// base + q2*j of work item u, base = (u / q2) * M + u % q2, q2 = M / 16.
+template <int N, int S>
+__device__ __forceinline__ void i16_pass2(uint32_t *data, uint32_t u, const uint32_t *twl, uint32_t sh1, uint32_t sh2)
+{
+   constexpr uint32_t M = (uint32_t) N >> (2 * S), q1 = M / 4u, q2 = M / 16u;
+   const uint32_t     d = u / q2, p2 = u % q2;
+   const uint32_t     base = d * M + p2;
+   uint32_t           v[16];
+#pragma unroll
+   for (int j = 0; j < 16; ++j) v[j] = data[i16_slot(base + q2 * (uint32_t) j)];
+#pragma unroll
+   for (int m = 0; m < 4; ++m)
+      i16_bfly4(v[m], v[m + 4], v[m + 8], v[m + 12], twl + i16_tw_off<N>(S), p2 + q2 * (uint32_t) m, q1, false, true,
+                sh1);
+#pragma unroll
+   for (int b = 0; b < 4; ++b)
+      i16_bfly4(v[4 * b], v[4 * b + 1], v[4 * b + 2], v[4 * b + 3], twl + i16_tw_off<N>(S + 1), p2, q2, q2 == 1u,
+                true, sh2);
+#pragma unroll
+   for (int j = 0; j < 16; ++j) data[i16_slot(base + q2 * (uint32_t) j)] = v[j];
+}
+
+// The last stage alone (block size 4, no twiddles) on points 16u .. 16u+15.
+__device__ __forceinline__ void i16_pass_last(uint32_t *data, uint32_t u, uint32_t sh)
+{
+   uint32_t v[16];
+#pragma unroll
+   for (int j = 0; j < 16; ++j) v[j] = data[i16_slot(16u * u + (uint32_t) j)];
+#pragma unroll
+   for (int b = 0; b < 4; ++b) i16_bfly4(v[4 * b], v[4 * b + 1], v[4 * b + 2], v[4 * b + 3], nullptr, 0u, 1u, true, true, sh);
+#pragma unroll
+   for (int j = 0; j < 16; ++j) data[i16_slot(16u * u + (uint32_t) j)] = v[j];
+}
+
+template <int N, int S>
+__device__ __forceinline__ void i16_passes_from(uint32_t *data, uint32_t u, bool live, const uint32_t *twl,
+                                                const uint32_t (&sh)[i16_log4(N)])
+{
+   constexpr int K = i16_log4(N);
+   if constexpr (S + 1 < K) {
+      if (live) i16_pass2<N, S>(data, u, twl, sh[S], sh[S + 1]);
+      __syncthreads();
+      i16_passes_from<N, S + 2>(data, u, live, twl, sh);
+   }
+   else if constexpr (S < K) {
+      if (live) i16_pass_last(data, u, sh[S]);
+      __syncthreads();
+   }
+}
+
+template <int N>
+constexpr size_t i16_pow4_lds()
+{
+   return ((size_t) (kThreads / (N / 16)) * (N + N / 16) + (size_t) (N - 1)) * sizeof(uint32_t);
+}
+
+template <int N>
+__global__ void __launch_bounds__(kThreads) i16_pow4_kernel(const uint4 *__restrict__ x, uint4 *__restrict__ y,
+                                                            const uint32_t *__restrict__ tw,
+                                                            const uint32_t *__restrict__ shift, uint32_t numChannels)
+{
+   constexpr int      K      = i16_log4(N);
+   constexpr int      T      = N / 16;
+   constexpr int      F      = kThreads / T;
+   constexpr uint32_t STRIDE = (uint32_t) (N + N / 16);
+   constexpr int      LOGN   = 2 * K;
+   extern __shared__ uint32_t smem16[];
+   uint32_t          *twl   = smem16 + F * STRIDE;
+   const uint32_t     first = blockIdx.x * (uint32_t) F;
+   const uint32_t     nf    = min((uint32_t) F, numChannels - first);
+   uint32_t           sh[K];
+#pragma unroll
+   for (int t = 0; t < K; ++t) sh[t] = shift[t];
+   for (uint32_t i = threadIdx.x; i < (uint32_t) (N - 1); i += kThreads) twl[i] = tw[i];
+   const uint4 *src = x + (size_t) first * (N / 4);
+   for (uint32_t i = threadIdx.x; i < nf * (uint32_t) (N / 4); i += kThreads) {
+      const uint4    v = src[i];
+      const uint32_t f = (4u * i) / (uint32_t) N;
+      const uint32_t k = 4u * i - f * (uint32_t) N;   // multiple of 4: slots k..k+3 are adjacent
+      uint32_t      *d = smem16 + f * STRIDE + i16_slot(k);
+      d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
+   }
+   __syncthreads();
+   const uint32_t fl = threadIdx.x / (uint32_t) T;
+   i16_passes_from<N, 0>(smem16 + fl * STRIDE, threadIdx.x % (uint32_t) T, fl < nf, twl, sh);
+   uint4 *dst = y + (size_t) first * (N / 4);
+   for (uint32_t i = threadIdx.x; i < nf * (uint32_t) (N / 4); i += kThreads) {
+      const uint32_t  f   = (4u * i) / (uint32_t) N;
+      const uint32_t  k   = 4u * i - f * (uint32_t) N;
+      const uint32_t *row = smem16 + f * STRIDE;
+      uint4           v;
+      v.x    = row[i16_slot(__brev(k) >> (32 - LOGN))];
+      v.y    = row[i16_slot(__brev(k + 1u) >> (32 - LOGN))];
+      v.z    = row[i16_slot(__brev(k + 2u) >> (32 - LOGN))];
+      v.w    = row[i16_slot(__brev(k + 3u) >> (32 - LOGN))];
+      dst[i] = v;
    }
 }
 
@@ -516,10 +651,31 @@ hipError_t launch_c2r_presplit(const float *x, float *z, const float *sf, uint32
    return hipGetLastError();
 }
 
+template <int N>
+hipError_t launch_i16_pow4(const int16_t *x, int16_t *y, const int16_t *tw, const uint32_t *shift,
+                           uint32_t numChannels, hipStream_t stream)
+{
+   constexpr uint32_t F = kThreads / (N / 16);
+   hipLaunchKernelGGL(i16_pow4_kernel<N>, dim3((numChannels + F - 1) / F), dim3(kThreads), i16_pow4_lds<N>(), stream,
+                      (const uint4 *) x, (uint4 *) y, (const uint32_t *) tw, shift, numChannels);
+   return hipGetLastError();
+}
+
 hipError_t launch_i16(const int16_t *x, int16_t *y, const int16_t *tw, const uint32_t *shift,
                       const StageTable &st, uint32_t numChannels, hipStream_t stream)
 {
    if (numChannels == 0) return hipSuccess;
+   // specialised lengths 4^2 .. 4^6 (rows of 16-byte multiples: 4 B per point)
+   if (((uintptr_t) x & 15u) == 0 && ((uintptr_t) y & 15u) == 0) {
+      switch (st.n) {
+      case 16: return launch_i16_pow4<16>(x, y, tw, shift, numChannels, stream);
+      case 64: return launch_i16_pow4<64>(x, y, tw, shift, numChannels, stream);
+      case 256: return launch_i16_pow4<256>(x, y, tw, shift, numChannels, stream);
+      case 1024: return launch_i16_pow4<1024>(x, y, tw, shift, numChannels, stream);
+      case 4096: return launch_i16_pow4<4096>(x, y, tw, shift, numChannels, stream);
+      default: break;
+      }
+   }
    Geometry g;
    if (!pick_geometry(st.n, 4u, st.twCount, numChannels, &g, false) || !g.twInLds) return hipErrorInvalidValue;
    return launch(i16_lds_kernel, g, numChannels, stream, (const uint32_t *) x, (uint32_t *) y,
