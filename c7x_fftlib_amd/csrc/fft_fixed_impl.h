@@ -909,19 +909,22 @@ __device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const 
 // each length gets its own register allocation (all lengths inlined into one
 // kernel body need several times the registers of the largest one). The task
 // and the kernel arguments are passed by address (kernel-argument or global
-// memory), the data through the kernel's dynamic LDS. Twiddles are read from
-// global memory (L2). Measured alternatives that were slower on MI355X:
-// twiddles staged in LDS (per group or per task), a loop over a task's groups
+// memory), the data through the kernel's dynamic LDS. Twiddle tables of at
+// most 511 entries are held in LDS across the groups of a task (N = 256:
+// 176 -> 166 us per 2^25 points); longer ones are read from global memory
+// (L2), since LDS room for them would cost occupancy. Measured alternatives
+// that were slower on MI355X: every table staged per group or per task, a loop over a task's groups
 // inside one call (its hoisted per-thread indexing spills at the register
 // budget), prefetching the next group's points into registers, and passing
 // the buffers as global-address-space pointers (global instead of flat
 // loads: 16% slower at N=256, 19% at N=4096).
+// Lengths whose twiddle table (at most 511 entries, 4 KB) the class kernel
+// keeps in LDS: staged when a workgroup's item belongs to a task with another
+// table than the one held, so consecutive groups of one task load it once.
 template <int N>
-__device__ __attribute__((noinline)) void class_group(const ClassTask *tk, const ClassArgs *c, uint32_t g)
+constexpr bool class_tw_lds()
 {
-   extern __shared__ float2 smem_class[];
-   const FixedArgs a = class_fixed_args(*c, *tk);
-   fixed_group<N, KIND_C2C>(a, g, smem_class);
+   return Fixed<N>::P.twCount <= 511;
 }
 
 template <int N>
@@ -930,12 +933,48 @@ constexpr size_t class_length_lds()
    return (size_t) Fixed<N>::P.FPB * class_stride<N>() * sizeof(float2);
 }
 
+// LDS of a class kernel, in float2 slots: the data region of its largest
+// length, then the twiddle region of its largest staged table, then one
+// slot holding the address of the table staged there.
 template <int... Ls>
-constexpr size_t class_lds_bytes()
+constexpr uint32_t class_data_slots()
 {
    size_t m = 0;
    ((m = m > class_length_lds<Ls>() ? m : class_length_lds<Ls>()), ...);
+   return (uint32_t) (m / sizeof(float2));
+}
+
+template <int... Ls>
+constexpr uint32_t class_tw_slots()
+{
+   uint32_t m = 0;
+   ((m = (class_tw_lds<Ls>() && (uint32_t) Fixed<Ls>::P.twCount > m) ? (uint32_t) Fixed<Ls>::P.twCount : m), ...);
    return m;
+}
+
+template <int... Ls>
+constexpr size_t class_lds_bytes()
+{
+   return ((size_t) class_data_slots<Ls...>() + class_tw_slots<Ls...>() + 1u) * sizeof(float2);
+}
+
+template <int N>
+__device__ __attribute__((noinline)) void class_group(const ClassTask *tk, const ClassArgs *c, uint32_t g,
+                                                      uint32_t twSlot, uint32_t heldSlot)
+{
+   extern __shared__ float2 smem_class[];
+   FixedArgs a = class_fixed_args(*c, *tk);
+   if constexpr (class_tw_lds<N>()) {
+      const float2 **held = (const float2 **) (smem_class + heldSlot);
+      float2        *twl  = smem_class + twSlot;
+      if (*held != a.tw) {   // uniform: one LDS word read by every work item
+         for (uint32_t i = threadIdx.x; i < (uint32_t) Fixed<N>::P.twCount; i += kFixedThreads) twl[i] = a.tw[i];
+         __syncthreads();
+         if (threadIdx.x == 0) *held = a.tw;   // read again only after this group's closing barrier
+      }
+      a.tw = twl;
+   }
+   fixed_group<N, KIND_C2C>(a, g, smem_class);
 }
 
 // Waves per SIMD the class's LDS allows (one wave of each resident
@@ -953,6 +992,11 @@ template <int... Ls>
 __global__ void __launch_bounds__(kFixedThreads) __attribute__((amdgpu_waves_per_eu(class_waves<Ls...>())))
 fixed_class_kernel(ClassArgs c)
 {
+   extern __shared__ float2 smem_class[];
+   constexpr uint32_t twSlot   = class_data_slots<Ls...>();
+   constexpr uint32_t heldSlot = twSlot + class_tw_slots<Ls...>();
+   if (threadIdx.x == 0) *(const float2 **) (smem_class + heldSlot) = nullptr;
+   __syncthreads();
    for (uint32_t w = blockIdx.x; w < c.numItems; w += gridDim.x) {
       uint32_t         g  = w;
       const ClassTask *tk = &c.single;
@@ -962,7 +1006,7 @@ fixed_class_kernel(ClassArgs c)
          g                   = item & ((1u << kClassGroupBits) - 1u);
       }
       const uint32_t n = tk->n;
-      (void) ((n == (uint32_t) Ls ? (class_group<Ls>(tk, &c, g), true) : false) || ...);
+      (void) ((n == (uint32_t) Ls ? (class_group<Ls>(tk, &c, g, twSlot, heldSlot), true) : false) || ...);
    }
 }
 
