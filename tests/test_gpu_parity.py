@@ -139,6 +139,40 @@ def test_i16_single_bit_exact(n, shifts):
     assert np.array_equal(y, oracle.i16(x, n, 1, shifts, single=True))
 
 
+@pytest.mark.parametrize("n", [16, 64, 256, 1024, 4096])
+def test_i16_specialised_and_generic_kernels_agree(n):
+    """N = 4^K runs the specialised kernel on 16-byte aligned buffers and the
+    one-stage-per-pass kernel on buffers 4 bytes off that; both bit-exact."""
+    import ctypes
+    import torch
+    F = G.F
+    k = int(round(np.log(n) / np.log(4)))
+    shifts = [1, 0, 1, 1, 0, 1][:k]
+    b = 37
+    x = _i16_input(rng(n + 5), b, n, -32768, 32767)
+    ref = oracle.i16(x, n, b, shifts)
+    L = G.lib()
+    assert L.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_GPU) == 0
+    L.FFTLIB_GPU_setSynchronous(1)
+    name = "FFTLIB_fft1dBatched_i16sc_c16sc_o16sc"
+    pb = G.pblock(name)
+    bx, bw = ctypes.byref(F.BufParams1D(F.FFTLIB_INT16, 2 * n * b)), ctypes.byref(F.BufParams1D(F.FFTLIB_INT16, 2 * n))
+    bs = ctypes.byref(F.BufParams1D(F.FFTLIB_UINT32, len(shifts)))
+    dw = torch.from_numpy(G.tw_i16(n)).cuda()
+    ds = torch.tensor(shifts, dtype=torch.int32, device="cuda")
+    for off in (0, 2):                      # int16 elements: 0 -> 16-byte aligned, 2 -> 4 bytes off
+        dx = torch.zeros(x.size + 8, dtype=torch.int16, device="cuda")
+        dx[off:off + x.size] = torch.from_numpy(x.reshape(-1)).cuda()
+        dy = torch.zeros_like(dx)
+        xp, yp = dx.data_ptr() + 2 * off, dy.data_ptr() + 2 * off
+        args = (xp, bx, dw.data_ptr(), bw, yp, bx, ds.data_ptr(), bs, n, b, pb)
+        assert getattr(L, name + "_init")(*args) == 0
+        assert getattr(L, name + "_kernel")(*args) == 0, F.last_error()
+        y = dy[off:off + x.size].cpu().numpy().reshape(b, n, 2)
+        assert np.array_equal(y, ref), (n, off)
+        assert not dy[:off].any() and not dy[off + x.size:].any()   # nothing written outside the rows
+
+
 R2C_SIZES = [(10, 16), (50, 4), (64, 1), (128, 2), (250, 4), (256, 4), (512, 8), (1024, 4), (1250, 4), (1600, 2),
              (4096, 3), (6250, 1), (8000, 2), (12800, 1), (16000, 1)]
 
