@@ -266,6 +266,13 @@ bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2)
    return true;
 }
 
+size_t two_pass_scratch_points(uint32_t n, uint32_t count)
+{
+   uint32_t n1 = 0, m2 = 0;
+   if (!two_pass_split(n, &n1, &m2)) return 0;
+   return (size_t) n1 * ((m2 + 15u) & ~15u) * count;
+}
+
 hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan)
 {
    uint32_t n1 = 0, m2 = 0;
@@ -296,7 +303,7 @@ hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan)
       return e;
    }
    TwoPassPlan p;
-   p.n = n, p.n1 = n1, p.m2 = m2;
+   p.n = n, p.n1 = n1, p.m2 = m2, p.s2 = (m2 + 15u) & ~15u;
    p.tw1 = d, p.tw2 = d + 2u * n1, p.colTw = d + 2u * ((size_t) n1 + m2);
    g_twoPass[TwoPassKey{dev, n}] = p;
    *plan = p;
@@ -308,7 +315,7 @@ ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const 
    ClassTask k{};
    const bool cols = layout == (int) LAYOUT_COLUMNS;
    k.n           = cols ? p.n1 : p.m2;
-   k.numChannels = count * (cols ? p.m2 : p.n1);
+   k.numChannels = count * (cols ? p.s2 : p.n1);   // columns m2..s2-1 of each transform idle
    k.numGroups   = (k.numChannels + fixed_fpb(k.n) - 1u) / fixed_fpb(k.n);
    k.layout      = (uint32_t) layout;
    k.tw          = (const float2 *) (cols ? p.tw1 : p.tw2);
@@ -316,6 +323,7 @@ ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const 
    k.outOff      = cols ? nullptr : off;
    k.scratchOff  = scratchOff;
    k.span        = cols ? p.m2 : p.n1;
+   k.scrStride   = p.s2;
    k.fullN       = p.n;
    k.colTw       = (const float2 *) p.colTw;
    return k;
@@ -330,7 +338,7 @@ hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t nu
    TwoPassPlan p;
    hipError_t  e = two_pass_plan(n, &p);
    if (e != hipSuccess) return e;
-   if ((uint64_t) numChannels * p.m2 >= (1ull << 32) || (uint64_t) numChannels * p.n1 >= (1ull << 32))
+   if ((uint64_t) numChannels * p.s2 >= (1ull << 32) || (uint64_t) numChannels * p.n1 >= (1ull << 32))
       return hipErrorInvalidValue;
    ClassArgs c{};
    c.x = (const float2 *) x, c.y = (float2 *) y, c.scratch = (float2 *) scratch, c.mode = mode;

@@ -36,14 +36,21 @@ uint32_t class_group_size(uint32_t n);
 // class-kernel lengths. The column pass (class of n1) runs the n1-point DFTs
 // of the m2 columns and applies W_n^(p*k1); the row pass (class of m2) runs
 // the m2-point DFTs of the n1 rows and writes the outputs in natural order.
+// The intermediate copy holds, per transform, n1 rows of s2 points (m2
+// rounded up to a multiple of 16, so that every row starts on a 128-byte
+// line; columns m2..s2-1 are not used).
 struct TwoPassPlan {
-   uint32_t     n, n1, m2;
+   uint32_t     n, n1, m2, s2;
    const float *tw1, *tw2;  // twiddle tables of n1 and m2 (device)
    const float *colTw;      // W_n^e, e < n (device)
 };
 
-// Most balanced split of n (false when n <= 16384, n > 2^22 or not 2/3/5-smooth).
+// Split of n (false when n <= 16384, n > 2^22 or not 2/3/5-smooth).
 bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2);
+
+// Complex points of the intermediate copy for `count` transforms of n points
+// (0 when n has no two-pass split).
+size_t two_pass_scratch_points(uint32_t n, uint32_t count);
 
 // Plan of n on the current device; its tables are built on first use and kept
 // until the process ends.
@@ -52,12 +59,13 @@ hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan);
 // Class-kernel task of one pass over `count` transforms: layout
 // LAYOUT_COLUMNS (off = input offsets of the transforms, or null) or
 // LAYOUT_ROWS (off = output offsets); scratchOff = complex offset of the
-// transforms' n*count-point region of the scratch buffer.
+// transforms' region of the scratch buffer (two_pass_scratch_points).
 fixedk::ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const uint64_t *off,
                                 uint64_t scratchOff);
 
 // numChannels contiguous transforms of n points, x -> y, through `scratch`
-// (n * numChannels complex values); mode as in launch_c2c.
+// (two_pass_scratch_points(n, numChannels) complex values); mode as in
+// launch_c2c.
 hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, float *scratch,
                                hipStream_t stream);
 

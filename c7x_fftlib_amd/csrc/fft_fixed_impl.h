@@ -229,6 +229,7 @@ struct FixedArgs {
    uint32_t      span;
    uint32_t      fullN;
    const float2 *colTw;       // LAYOUT_COLUMNS: W_fullN^e for e < fullN
+   uint32_t      scrStride;   // two-pass: points per row of the intermediate copy (span of the columns rounded up to 16)
 };
 
 enum : uint32_t { LAYOUT_PLAIN = 0u, LAYOUT_COLUMNS = 1u, LAYOUT_ROWS = 2u };
@@ -715,6 +716,7 @@ struct ClassTask {
    uint64_t        scratchOff;   // two-pass: complex offset of the task's scratch region
    uint32_t        span, fullN;  // two-pass: see FixedArgs
    const float2   *colTw;
+   uint32_t        scrStride;    // two-pass: see FixedArgs
 };
 
 struct ClassArgs {
@@ -741,7 +743,10 @@ constexpr uint32_t class_stride()
 
 // Column group: the FPB adjacent columns of the group are loaded row by row
 // (lane j of a row takes column j), transformed in LDS, multiplied by
-// W_fullN^(p*k1) and stored row by row.
+// W_fullN^(p*k1) and stored row by row into the intermediate copy, whose
+// rows have scrStride points. Channels number scrStride columns per
+// transform so that a group never straddles two transforms; columns from
+// span on are idle (they read column span-1 and store nothing).
 template <int N>
 __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, float2 *smem)
 {
@@ -754,11 +759,11 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
    const uint32_t      j      = threadIdx.x % FPB;
    const uint32_t      i0     = threadIdx.x / FPB;
    const uint32_t      ch     = g * FPB + j;
-   const bool          live   = ch < a.numChannels;
-   const uint32_t      chc    = live ? ch : g * FPB;   // lanes past the batch re-read the first column
-   const uint32_t      f      = chc / a.span;
-   const uint32_t      p      = chc - f * a.span;
-   const float2       *in     = a.x + (a.inOff ? (size_t) a.inOff[f] : (size_t) f * a.fullN) + p;
+   const uint32_t      chc    = ch < a.numChannels ? ch : g * FPB;   // lanes past the batch re-read the first column
+   const uint32_t      f      = chc / a.scrStride;
+   const uint32_t      p      = chc - f * a.scrStride;
+   const bool          live   = ch < a.numChannels && p < a.span;
+   const float2       *in     = a.x + (a.inOff ? (size_t) a.inOff[f] : (size_t) f * a.fullN) + min(p, a.span - 1u);
    const InScale       sc     = in_scale(a.mode, 1.0f / (float) a.fullN);
    float2             *col    = smem + j * STRIDE;
 #pragma unroll 1
@@ -779,7 +784,7 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
    __syncthreads();
    fixed_passes_from<N, 0>(smem + (threadIdx.x / T) * STRIDE, threadIdx.x % T, a.tw);
    if (live) {
-      float2 *out = a.y + (size_t) f * a.fullN + p;
+      float2 *out = a.y + (size_t) f * ((size_t) N * a.scrStride) + p;
 #pragma unroll 1
       for (int c0 = 0; c0 < IT; c0 += CH) {
          float2 v[CH], w[CH];
@@ -793,7 +798,7 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = i0 + (uint32_t) ((c0 + c) * T);
             if ((N % T == 0 && IT % CH == 0) || (c0 + c < IT && k < (uint32_t) N))
-               out[(size_t) a.span * k] = make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
+               out[(size_t) a.scrStride * k] = make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
                                                       v[c].x * w[c].y + v[c].y * w[c].x);
          }
       }
@@ -894,7 +899,7 @@ __device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const 
    a.numChannels = k.numChannels;
    a.numGroups   = k.numGroups;
    a.mode        = c.mode;
-   a.inStride    = k.n;
+   a.inStride    = k.layout == LAYOUT_ROWS ? k.scrStride : k.n;   // rows of the intermediate copy
    a.outStride   = k.n;
    a.inOff       = k.layout == LAYOUT_ROWS ? nullptr : k.inOff;
    a.outOff      = k.layout == LAYOUT_COLUMNS ? nullptr : k.outOff;
@@ -902,6 +907,7 @@ __device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const 
    a.span        = k.span;
    a.fullN       = k.fullN;
    a.colTw       = k.colTw;
+   a.scrStride   = k.scrStride;
    return a;
 }
 

@@ -94,7 +94,7 @@ hipError_t run_complex(const StageTable &st, const float *x, float *y, const flo
    uint32_t n1 = 0, m2 = 0;
    if (specialized_kernels() && two_pass_split(st.n, &n1, &m2)) {
       void      *scratch = nullptr;
-      hipError_t e       = device_scratch((size_t) st.n * numChannels * 8u, s, SCRATCH_COMPLEX, &scratch);
+      hipError_t e       = device_scratch(two_pass_scratch_points(st.n, numChannels) * 8u, s, SCRATCH_COMPLEX, &scratch);
       if (e != hipSuccess) return e;
       return launch_c2c_two_pass(x, y, st.n, numChannels, mode, (float *) scratch, s);
    }

@@ -329,7 +329,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
             return fail("ragged two-pass tables", e);
          }
          g.scrOff = scr / sizeof(float2);
-         scr += ((size_t) g.n * g.count * 8u + 255u) & ~(size_t) 255u;
+         scr += (two_pass_scratch_points(g.n, g.count) * 8u + 255u) & ~(size_t) 255u;
       }
       if ((e = hipMalloc((void **) &g.dOff, kv.second.size() * sizeof(uint64_t))) == hipSuccess)
          e = hipMemcpy(g.dOff, kv.second.data(), kv.second.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
