@@ -286,15 +286,28 @@ hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan)
       *plan = it->second;
       return hipSuccess;
    }
-   // [table of n1 | table of m2 | W_n^e, e < n], complex fp32 from double
-   std::vector<float> host(2u * ((size_t) n1 + m2 + n));
+   // [table of n1 | table of m2 | W_n^e for e < L | W_n^(h L) for h < ceil(n / L)],
+   // L = 2^bits ~ sqrt(n): the column twiddle W_n^e is the product of one
+   // entry of each of the last two tables (a few KB per length, so the
+   // tables of many lengths stay in L2), complex fp32 from double
+   uint32_t bits = 0;
+   while ((1ull << (2u * bits)) < n) ++bits;
+   const uint32_t     L  = 1u << bits;
+   const uint32_t     nh = (n + L - 1u) / L;
+   std::vector<float> host(2u * ((size_t) n1 + m2 + L + nh));
    FFTLIB_tw_gen_c2c(host.data(), (int32_t) n1);
    FFTLIB_tw_gen_c2c(host.data() + 2u * n1, (int32_t) m2);
    float *col = host.data() + 2u * ((size_t) n1 + m2);
-   for (uint32_t k = 0; k < n; ++k) {
+   for (uint32_t k = 0; k < L; ++k) {
       const double a = 2.0 * M_PI * (double) k / (double) n;
       col[2u * k]      = (float) cos(a);
       col[2u * k + 1u] = (float) -sin(a);
+   }
+   float *colHi = col + 2u * L;
+   for (uint32_t h = 0; h < nh; ++h) {
+      const double a = 2.0 * M_PI * (double) ((uint64_t) h * L) / (double) n;
+      colHi[2u * h]      = (float) cos(a);
+      colHi[2u * h + 1u] = (float) -sin(a);
    }
    float *d = nullptr;
    if ((e = hipMalloc((void **) &d, host.size() * sizeof(float))) != hipSuccess) return e;
@@ -305,6 +318,7 @@ hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan)
    TwoPassPlan p;
    p.n = n, p.n1 = n1, p.m2 = m2, p.s2 = (m2 + 15u) & ~15u;
    p.tw1 = d, p.tw2 = d + 2u * n1, p.colTw = d + 2u * ((size_t) n1 + m2);
+   p.colTwHi = p.colTw + 2u * L, p.colBits = bits;
    g_twoPass[TwoPassKey{dev, n}] = p;
    *plan = p;
    return hipSuccess;
@@ -326,6 +340,8 @@ ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const 
    k.scrStride   = p.s2;
    k.fullN       = p.n;
    k.colTw       = (const float2 *) p.colTw;
+   k.colTwHi     = (const float2 *) p.colTwHi;
+   k.colBits     = p.colBits;
    return k;
 }
 

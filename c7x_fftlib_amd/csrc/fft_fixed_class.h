@@ -42,7 +42,9 @@ uint32_t class_group_size(uint32_t n);
 struct TwoPassPlan {
    uint32_t     n, n1, m2, s2;
    const float *tw1, *tw2;  // twiddle tables of n1 and m2 (device)
-   const float *colTw;      // W_n^e, e < n (device)
+   const float *colTw;      // W_n^e for e < 2^colBits (device)
+   const float *colTwHi;    // W_n^(h 2^colBits) for h < ceil(n / 2^colBits) (device)
+   uint32_t     colBits;    // 2^colBits ~ sqrt(n)
 };
 
 // Split of n (false when n <= 16384, n > 2^22 or not 2/3/5-smooth).

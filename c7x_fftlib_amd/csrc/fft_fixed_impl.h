@@ -228,7 +228,9 @@ struct FixedArgs {
    uint32_t      layout;
    uint32_t      span;
    uint32_t      fullN;
-   const float2 *colTw;       // LAYOUT_COLUMNS: W_fullN^e for e < fullN
+   const float2 *colTw;       // LAYOUT_COLUMNS: W_fullN^e for e < 2^colBits ...
+   const float2 *colTwHi;     // ... and W_fullN^(h 2^colBits): W^e = colTwHi[e >> colBits] colTw[e & mask]
+   uint32_t      colBits;
    uint32_t      scrStride;   // two-pass: points per row of the intermediate copy (span of the columns rounded up to 16)
 };
 
@@ -716,6 +718,8 @@ struct ClassTask {
    uint64_t        scratchOff;   // two-pass: complex offset of the task's scratch region
    uint32_t        span, fullN;  // two-pass: see FixedArgs
    const float2   *colTw;
+   const float2   *colTwHi;
+   uint32_t        colBits;
    uint32_t        scrStride;    // two-pass: see FixedArgs
 };
 
@@ -743,7 +747,8 @@ constexpr uint32_t class_stride()
 
 // Column group: the FPB adjacent columns of the group are loaded row by row
 // (lane j of a row takes column j), transformed in LDS, multiplied by
-// W_fullN^(p*k1) and stored row by row into the intermediate copy, whose
+// W_fullN^(p*k1) (product of two small-table entries) and stored row by row
+// into the intermediate copy, whose
 // rows have scrStride points. Channels number scrStride columns per
 // transform so that a group never straddles two transforms; columns from
 // span on are idle (they read column span-1 and store nothing).
@@ -792,7 +797,9 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = min(i0 + (uint32_t) ((c0 + c) * T), (uint32_t) N - 1u);
             v[c]             = col[fslot(fixed_position<N>(k))];
-            w[c]             = a.colTw[p * k];
+            const uint32_t e = p * k;   // < fullN
+            const float2   h = a.colTwHi[e >> a.colBits], l = a.colTw[e & ((1u << a.colBits) - 1u)];
+            w[c]             = make_float2(h.x * l.x - h.y * l.y, h.x * l.y + h.y * l.x);
          }
 #pragma unroll
          for (int c = 0; c < CH; ++c) {
@@ -907,6 +914,8 @@ __device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const 
    a.span        = k.span;
    a.fullN       = k.fullN;
    a.colTw       = k.colTw;
+   a.colTwHi     = k.colTwHi;
+   a.colBits     = k.colBits;
    a.scrStride   = k.scrStride;
    return a;
 }
