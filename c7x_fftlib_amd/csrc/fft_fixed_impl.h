@@ -735,6 +735,7 @@ struct ClassArgs {
 };
 
 constexpr uint32_t kClassGroupBits = 20;
+constexpr uint32_t kXcds           = 8;   // accelerator dies (XCDs) of an MI355X, each with its own L2
 
 // Per-transform LDS stride of the class kernels: odd, so that the FPB
 // channels of a column or row group (lanes j = 0..FPB-1 at the same point)
@@ -1012,7 +1013,20 @@ fixed_class_kernel(ClassArgs c)
    constexpr uint32_t heldSlot = twSlot + class_tw_slots<Ls...>();
    if (threadIdx.x == 0) *(const float2 **) (smem_class + heldSlot) = nullptr;
    __syncthreads();
-   for (uint32_t w = blockIdx.x; w < c.numItems; w += gridDim.x) {
+   // Items in XCD-contiguous order: workgroups are dispatched round robin over
+   // the kXcds dies (workgroup w on die w % kXcds), so the items are dealt in
+   // kXcds contiguous runs, one per die. Adjacent groups share the cache lines
+   // at their edges when rows or columns are not line aligned; this way they
+   // meet in the same L2.
+   uint32_t first = blockIdx.x, end = c.numItems, step = gridDim.x;
+   if (gridDim.x % kXcds == 0u && c.numItems >= gridDim.x) {
+      const uint32_t per   = (c.numItems + kXcds - 1u) / kXcds;
+      const uint32_t begin = min((blockIdx.x % kXcds) * per, c.numItems);
+      end                  = min(begin + per, c.numItems);
+      first                = begin + blockIdx.x / kXcds;
+      step                 = gridDim.x / kXcds;
+   }
+   for (uint32_t w = first; w < end; w += step) {
       uint32_t         g  = w;
       const ClassTask *tk = &c.single;
       if (c.tasks) {
