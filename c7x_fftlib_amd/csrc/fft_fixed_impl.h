@@ -898,7 +898,10 @@ __device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, floa
    __syncthreads();
 }
 
-__device__ __forceinline__ FixedArgs class_fixed_args(const ClassArgs &c, const ClassTask &k)
+// C and K: ClassArgs and ClassTask, or the same read through the constant
+// address space (scalar loads).
+template <class C, class K>
+__device__ __forceinline__ FixedArgs class_fixed_args(const C &c, const K &k)
 {
    FixedArgs a{};
    a.x           = k.layout == LAYOUT_ROWS ? c.scratch + k.scratchOff : c.x;
@@ -974,16 +977,39 @@ constexpr size_t class_lds_bytes()
    return ((size_t) class_data_slots<Ls...>() + class_tw_slots<Ls...>() + 1u) * sizeof(float2);
 }
 
+// Arguments of a called function arrive in vector registers; the class
+// kernel's are the same in every work item, and taking them (and the task
+// addresses) back into scalar registers lets the task and the kernel
+// arguments be read with scalar loads and keeps the control flow uniform.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v)
+{
+   return ((uint64_t) wave_uniform((uint32_t) (v >> 32)) << 32) | wave_uniform((uint32_t) v);
+}
+using ClassArgsK = const __attribute__((address_space(4))) ClassArgs;
+using ClassTaskK = const __attribute__((address_space(4))) ClassTask;
+
+// task: address of the ClassTask (in the task table, or the kernel argument's
+// `single`); args: address of the kernel argument (its kernarg segment).
+// not_tail_called: a call marked as a tail call makes the callee save and
+// restore the callee-saved registers it uses (up to 50 VGPRs, 20-30% of a
+// group's time); with plain calls from the one kernel the compiler leaves
+// them unsaved and the kernel does not rely on them.
 template <int N>
-__device__ __attribute__((noinline)) void class_group(const ClassTask *tk, const ClassArgs *c, uint32_t g,
-                                                      uint32_t twSlot, uint32_t heldSlot)
+__device__ __attribute__((noinline, not_tail_called)) void class_group(uint64_t task, uint64_t args, uint32_t g, uint32_t twSlot,
+                                                      uint32_t heldSlot)
 {
    extern __shared__ float2 smem_class[];
-   FixedArgs a = class_fixed_args(*c, *tk);
+   ClassTaskK *tk = (ClassTaskK *) wave_uniform(task);
+   ClassArgsK *c  = (ClassArgsK *) wave_uniform(args);
+   g              = wave_uniform(g);
+   twSlot         = wave_uniform(twSlot);
+   heldSlot       = wave_uniform(heldSlot);
+   FixedArgs a    = class_fixed_args(*c, *tk);
    if constexpr (class_tw_lds<N>()) {
       const float2 **held = (const float2 **) (smem_class + heldSlot);
       float2        *twl  = smem_class + twSlot;
-      if (*held != a.tw) {   // uniform: one LDS word read by every work item
+      if ((const float2 *) wave_uniform((uint64_t) *held) != a.tw) {   // one LDS word read by every work item
          for (uint32_t i = threadIdx.x; i < (uint32_t) Fixed<N>::P.twCount; i += kFixedThreads) twl[i] = a.tw[i];
          __syncthreads();
          if (threadIdx.x == 0) *held = a.tw;   // read again only after this group's closing barrier
@@ -1026,16 +1052,19 @@ fixed_class_kernel(ClassArgs c)
       first                = begin + blockIdx.x / kXcds;
       step                 = gridDim.x / kXcds;
    }
+   // c is the kernel's only argument: at offset 0 of the kernarg segment.
+   const uint64_t args = (uint64_t) __builtin_amdgcn_kernarg_segment_ptr();
    for (uint32_t w = first; w < end; w += step) {
-      uint32_t         g  = w;
-      const ClassTask *tk = &c.single;
+      uint32_t g    = w;
+      uint64_t task = args + offsetof(ClassArgs, single);
+      uint32_t n    = c.single.n;
       if (c.tasks) {
          const uint32_t item = c.items[w];
-         tk                  = c.tasks + (item >> kClassGroupBits);
+         task                = (uint64_t) (c.tasks + (item >> kClassGroupBits));
          g                   = item & ((1u << kClassGroupBits) - 1u);
+         n                   = c.tasks[item >> kClassGroupBits].n;
       }
-      const uint32_t n = tk->n;
-      (void) ((n == (uint32_t) Ls ? (class_group<Ls>(tk, &c, g, twSlot, heldSlot), true) : false) || ...);
+      (void) ((n == (uint32_t) Ls ? (class_group<Ls>(task, args, g, twSlot, heldSlot), true) : false) || ...);
    }
 }
 
