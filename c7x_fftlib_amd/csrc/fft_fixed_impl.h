@@ -928,15 +928,19 @@ __device__ __forceinline__ FixedArgs class_fixed_args(const C &c, const K &k)
 // each length gets its own register allocation (all lengths inlined into one
 // kernel body need several times the registers of the largest one). The task
 // and the kernel arguments are passed by address (kernel-argument or global
-// memory), the data through the kernel's dynamic LDS. Twiddle tables of at
-// most 511 entries are held in LDS across the groups of a task (N = 256:
-// 176 -> 166 us per 2^25 points); longer ones are read from global memory
-// (L2), since LDS room for them would cost occupancy. Measured alternatives
-// that were slower on MI355X: every table staged per group or per task, a loop over a task's groups
-// inside one call (its hoisted per-thread indexing spills at the register
-// budget), prefetching the next group's points into registers, and passing
-// the buffers as global-address-space pointers (global instead of flat
-// loads: 16% slower at N=256, 19% at N=4096).
+// memory) and read through the constant address space: scalar loads, uniform
+// branches, and buffer pointers the compiler knows to be global (global
+// instead of flat loads and stores, which would also count against the LDS
+// wait counter). Passing the address of a stack copy of the kernel argument
+// instead made every field a per-lane flat load (N = 4096: 149 -> 126 us per
+// 2^25 points). The data goes through the kernel's dynamic LDS. Twiddle
+// tables of at most 511 entries are held in LDS across the groups of a task
+// (N = 256: 176 -> 166 us per 2^25 points); longer ones are read from global
+// memory (L2), since LDS room for them would cost occupancy. Measured
+// alternatives that were slower on MI355X: every table staged per group or
+// per task, a loop over a task's groups inside one call (its hoisted
+// per-thread indexing spills at the register budget), and prefetching the
+// next group's points into registers (measured with per-lane arguments).
 // Lengths whose twiddle table (at most 511 entries, 4 KB) the class kernel
 // keeps in LDS: staged when a workgroup's item belongs to a task with another
 // table than the one held, so consecutive groups of one task load it once.
