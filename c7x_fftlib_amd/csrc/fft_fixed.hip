@@ -117,20 +117,20 @@ int fixed_class_of(uint32_t n)
    return k;
 }
 
-hipError_t fixed_class_launch(int cls, const ClassArgs &c, uint32_t maxGrid, hipStream_t stream)
+hipError_t fixed_class_launch(int cls, const ClassArgs &c, uint32_t grid, hipStream_t stream)
 {
    switch (cls) {
-   case 0: return fixed_class_launch_tu1(cls, c, maxGrid, stream);
-   case 1: return fixed_class_launch_tu0(cls, c, maxGrid, stream);
-   case 2: return fixed_class_launch_tu2(cls, c, maxGrid, stream);
-   case 3: return fixed_class_launch_tu3(cls, c, maxGrid, stream);
-   case 4: return fixed_class_launch_tu1(cls, c, maxGrid, stream);
-   case 5: return fixed_class_launch_tu2(cls, c, maxGrid, stream);
-   case 6: return fixed_class_launch_tu3(cls, c, maxGrid, stream);
-   case 7: return fixed_class_launch_tu3(cls, c, maxGrid, stream);
-   case 8: return fixed_class_launch_tu2(cls, c, maxGrid, stream);
-   case 9: return fixed_class_launch_tu1(cls, c, maxGrid, stream);
-   case 10: return fixed_class_launch_tu0(cls, c, maxGrid, stream);
+   case 0: return fixed_class_launch_tu1(cls, c, grid, stream);
+   case 1: return fixed_class_launch_tu0(cls, c, grid, stream);
+   case 2: return fixed_class_launch_tu2(cls, c, grid, stream);
+   case 3: return fixed_class_launch_tu3(cls, c, grid, stream);
+   case 4: return fixed_class_launch_tu1(cls, c, grid, stream);
+   case 5: return fixed_class_launch_tu2(cls, c, grid, stream);
+   case 6: return fixed_class_launch_tu3(cls, c, grid, stream);
+   case 7: return fixed_class_launch_tu3(cls, c, grid, stream);
+   case 8: return fixed_class_launch_tu2(cls, c, grid, stream);
+   case 9: return fixed_class_launch_tu1(cls, c, grid, stream);
+   case 10: return fixed_class_launch_tu0(cls, c, grid, stream);
    default: return hipErrorInvalidValue;
    }
 }
@@ -139,12 +139,12 @@ namespace {
 
 // Complex FFT of length n on its specialised kernel: the dedicated one for
 // the lengths listed above, else its class kernel with one task.
-hipError_t dispatch_c2c_any(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hipStream_t stream)
+hipError_t dispatch_c2c_any(uint32_t n, const FixedArgs &a, hipStream_t stream)
 {
    if (main_c2c_has(n)) {
       switch (n) {
 #define FFTLIB_FIXED_CASE(L)                                                                                        \
-   case L: return launch_fixed<L, KIND_C2C>(a, stream, maxGrid);
+   case L: return launch_fixed<L, KIND_C2C>(a, stream);
          FFTLIB_FIXED_C2C_LENGTHS(FFTLIB_FIXED_CASE)
 #undef FFTLIB_FIXED_CASE
       default: break;
@@ -156,7 +156,7 @@ hipError_t dispatch_c2c_any(uint32_t n, const FixedArgs &a, uint32_t maxGrid, hi
    c.x = a.x, c.y = a.y, c.scratch = nullptr, c.mode = a.mode;
    c.single   = single_task(n, a.numChannels, a.tw, a.inOff, a.outOff);
    c.numItems = c.single.numGroups;
-   return fixed_class_launch(cls, c, maxGrid, stream);
+   return fixed_class_launch(cls, c, 0, stream);
 }
 
 }  // namespace
@@ -181,7 +181,7 @@ hipError_t launch_c2c_fixed(const float *x, float *y, const float *tw, uint32_t 
    FixedArgs a{};
    a.x = (const float2 *) x, a.y = (float2 *) y, a.tw = (const float2 *) tw, a.sf = nullptr;
    a.numChannels = numChannels, a.mode = mode, a.aux = 0, a.inStride = n, a.outStride = n;
-   return dispatch_c2c_any(n, a, 0, stream);
+   return dispatch_c2c_any(n, a, stream);
 }
 
 hipError_t launch_r2c_fixed(const float *x, float *y, const float *tw, const float *sf, uint32_t nc,

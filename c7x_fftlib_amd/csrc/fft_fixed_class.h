@@ -19,15 +19,16 @@ constexpr int kFixedClasses = 11;
 // Class of length n (-1 when n has none).
 int fixed_class_of(uint32_t n);
 
-// Launches class cls over c's items; maxGrid > 0 caps the persistent grid.
-hipError_t fixed_class_launch(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
+// Launches class cls over c's items; grid > 0 sets the number of workgroups
+// (at most the item count), 0 = the class kernels' default.
+hipError_t fixed_class_launch(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
 
 // Per translation unit: launch of the classes it instantiates
 // (hipErrorInvalidValue for others).
-hipError_t fixed_class_launch_tu0(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
-hipError_t fixed_class_launch_tu1(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
-hipError_t fixed_class_launch_tu2(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
-hipError_t fixed_class_launch_tu3(int cls, const fixedk::ClassArgs &c, uint32_t maxGrid, hipStream_t stream);
+hipError_t fixed_class_launch_tu0(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
+hipError_t fixed_class_launch_tu1(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
+hipError_t fixed_class_launch_tu2(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
+hipError_t fixed_class_launch_tu3(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
 
 // Workgroup group size (transforms per workgroup) of length n's class kernel.
 uint32_t class_group_size(uint32_t n);

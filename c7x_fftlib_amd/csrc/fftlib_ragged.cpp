@@ -59,7 +59,7 @@ struct FFTLIB_GPU_ragged_s {
       uint32_t                numItems = 0;
       fixedk::ClassTask      *dTasks   = nullptr;
       uint32_t               *dItems   = nullptr;
-      uint32_t                maxGrid  = 0;
+      uint32_t                grid     = 0;   // workgroups of the launch (0: the class default)
       double                  work     = 0.0;
    };
    std::vector<Class> classes;   // phase 0 first, then phase 1; by work within a phase
@@ -91,8 +91,8 @@ bool smooth235(uint32_t n)
 
 // ---- tuning cache -------------------------------------------------------------
 // Key: the class-kernel launch (kClassKeyBase + 100 * phase + octave index)
-// and a power-of-two bucket of its item count; value: the persistent-grid cap
-// and its time.
+// and a power-of-two bucket of its item count; value: the launch's number of
+// workgroups (0 = the class kernels' default) and its time.
 constexpr uint32_t kClassKeyBase = 100000u;
 uint32_t class_key(const FFTLIB_GPU_ragged_s::Class &c) { return kClassKeyBase + 100u * (uint32_t) c.phase + (uint32_t) c.cls; }
 struct TuneKey {
@@ -100,7 +100,7 @@ struct TuneKey {
    bool     operator<(const TuneKey &o) const { return n != o.n ? n < o.n : bucket < o.bucket; }
 };
 struct TuneVal {
-   uint32_t maxGrid;
+   uint32_t grid;
    float    us;
 };
 std::mutex                     g_tuneMu;
@@ -113,22 +113,22 @@ uint32_t bucket_of(uint32_t count)
    return b;
 }
 
-bool tuned(uint32_t n, uint32_t count, uint32_t *maxGrid)
+bool tuned(uint32_t n, uint32_t count, uint32_t *grid)
 {
    std::lock_guard<std::mutex> lock(g_tuneMu);
    auto it = g_tune.find(TuneKey{n, bucket_of(count)});
    if (it == g_tune.end()) return false;
-   *maxGrid = it->second.maxGrid;
+   *grid = it->second.grid;
    return true;
 }
 
 hipError_t launch_class(const FFTLIB_GPU_ragged_s &h, const FFTLIB_GPU_ragged_s::Class &c, const float *in, float *out,
-                        int mode, uint32_t maxGrid, hipStream_t s)
+                        int mode, uint32_t grid, hipStream_t s)
 {
    fixedk::ClassArgs a{};
    a.x = (const float2 *) in, a.y = (float2 *) out, a.scratch = h.scratch, a.mode = mode;
    a.numItems = c.numItems, a.tasks = c.dTasks, a.items = c.dItems;
-   return fixed_class_launch(c.cls, a, maxGrid, s);
+   return fixed_class_launch(c.cls, a, grid, s);
 }
 
 FFTLIB_STATUS fail(const char *what, hipError_t e)
@@ -190,7 +190,7 @@ hipError_t enqueue_direction(FFTLIB_GPU_ragged_s *h, const float *in, float *out
       size_t next = 0;
       for (const auto &c : h->classes) {
          if (c.phase != phase) continue;
-         if ((e = launch_class(*h, c, in, out, mode, c.maxGrid, h->lanes[next++ % h->lanes.size()])) != hipSuccess)
+         if ((e = launch_class(*h, c, in, out, mode, c.grid, h->lanes[next++ % h->lanes.size()])) != hipSuccess)
             return e;
       }
       if ((e = join_lanes(h, origin, ev)) != hipSuccess) return e;
@@ -285,7 +285,7 @@ hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
           (e = hipMemcpy(h->classes.back().dItems, items.data(), items.size() * sizeof(uint32_t),
                          hipMemcpyHostToDevice)) != hipSuccess)
          return e;
-      (void) tuned(class_key(h->classes.back()), c.numItems, &h->classes.back().maxGrid);
+      (void) tuned(class_key(h->classes.back()), c.numItems, &h->classes.back().grid);
    }
    std::sort(h->classes.begin(), h->classes.end(),
              [](const FFTLIB_GPU_ragged_s::Class &a, const FFTLIB_GPU_ragged_s::Class &b) {
@@ -453,8 +453,10 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    // results of the same input
    for (auto &c : handle->classes) {
       if (e != hipSuccess) break;
-      // candidates: resident slots (0), then caps at multiples of the CU count
-      const uint32_t cand[] = {0u, 4u * (uint32_t) cus, 2u * (uint32_t) cus, (uint32_t) cus, (uint32_t) cus / 2u};
+      // candidates: the default (0: up to 8 x the resident workgroups), then
+      // multiples of the CU count, each at most the launch's item count
+      const uint32_t u      = (uint32_t) cus;
+      const uint32_t cand[] = {0u, 32u * u, 16u * u, 8u * u, 4u * u, 2u * u, u, u / 2u};
       float          best   = 1e30f;
       uint32_t       bestG  = 0;
       for (uint32_t cap : cand) {
@@ -469,7 +471,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
          if (ms < best) best = ms, bestG = cap;
       }
       if (e != hipSuccess) break;
-      c.maxGrid = bestG;
+      c.grid = bestG;
       std::lock_guard<std::mutex> tl(g_tuneMu);
       g_tune[TuneKey{class_key(c), bucket_of(c.numItems)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
    }
@@ -490,7 +492,7 @@ FFTLIB_STATUS FFTLIB_GPU_tuneSave(const char *path)
    std::lock_guard<std::mutex> lock(g_tuneMu);
    fprintf(f, "fftlib-amd-tune 1\n");
    for (const auto &kv : g_tune)
-      fprintf(f, "%u %u %u %.3f\n", kv.first.n, kv.first.bucket, kv.second.maxGrid, (double) kv.second.us);
+      fprintf(f, "%u %u %u %.3f\n", kv.first.n, kv.first.bucket, kv.second.grid, (double) kv.second.us);
    const bool ok = fclose(f) == 0;
    return ok ? FFTLIB_SUCCESS : FFTLIB_ERR_FAILURE;
 }
