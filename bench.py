@@ -521,8 +521,8 @@ def load_traffic(workload, kernel):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the batch is described in "
-                                   "DESIGN.md but no multi-GPU path is built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the batched FFT has no multi-GPU path "
+                                   "(a batch would be split by channels, one share per GPU)", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
