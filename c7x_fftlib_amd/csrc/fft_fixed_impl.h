@@ -481,6 +481,76 @@ constexpr inline int largest_divisor_upto(int n, int cap)
    return c;
 }
 
+// R2C store of a whole group when a workgroup holds FPB > 1 transforms: work
+// item w handles outputs i = w, w + 256, ... of every transform of the group,
+// so the split factors A[i], B[i] and the digit-reversed slots of i and N - i
+// are loaded and computed once per output index instead of once per
+// transform (R2C 2048, 503 MB per call: 233.7 -> 205.3 us; 960: 231.6 ->
+// 202.2 us). Same formula per output as fixed_store; the compiler may contract its
+// products into FMAs differently, so the two can differ in the last bit. Used from
+// N = 256 on, where every work item has outputs; shorter lengths would leave
+// most work items idle while the others loop over many transforms.
+template <int N>
+constexpr bool r2c_group_store()
+{
+   return Fixed<N>::P.FPB > 1 && N >= kFixedThreads;
+}
+
+template <int N>
+__device__ __forceinline__ void r2c_store_group(const float2 *smem, uint32_t first, uint32_t nf, const FixedArgs &a)
+{
+   constexpr int      FPB    = Fixed<N>::P.FPB;
+   constexpr uint32_t STRIDE = fixed_stride<N>();
+   constexpr int      iters  = (N + kFixedThreads - 1) / kFixedThreads;
+   constexpr int      CH     = largest_divisor_upto(iters, 4);
+#pragma unroll 1
+   for (int c0 = 0; c0 < iters; c0 += CH) {
+      float2   A[CH], B[CH];
+      uint32_t s[CH], sm[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+         const uint32_t i = min(threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads), (uint32_t) N - 1u);
+         A[c]             = split_factor(a, i);
+         B[c]             = split_factor(a, 2u * N - 1u - i);
+         s[c]             = fslot(fixed_position<N>(i));
+         sm[c]            = fslot(fixed_position<N>(i == 0u ? 0u : (uint32_t) N - i));
+      }
+#pragma unroll
+      for (int f = 0; f < FPB; ++f) {
+         if ((uint32_t) f >= nf) break;
+         const float2 *src = smem + f * STRIDE;
+         float2       *out = a.y + out_base(a, first + (uint32_t) f);
+         float2        zi[CH], zm[CH];
+#pragma unroll
+         for (int c = 0; c < CH; ++c) zi[c] = src[s[c]], zm[c] = src[sm[c]];
+#pragma unroll
+         for (int c = 0; c < CH; ++c) {
+            const uint32_t i = threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads);
+            if (N % kFixedThreads != 0 && i >= (uint32_t) N) break;
+            float2 t1;
+            if (i == 0u) t1 = make_float2(zi[c].x * B[c].x + zi[c].y * B[c].y, zi[c].x * B[c].x - zi[c].y * B[c].y);
+            else t1 = make_float2(zm[c].x * B[c].x + zm[c].y * B[c].y, zm[c].x * B[c].y - zm[c].y * B[c].x);
+            const float2 y = make_float2(zi[c].x * A[c].x - zi[c].y * A[c].y + t1.x,
+                                         zi[c].x * A[c].y + zi[c].y * A[c].x + t1.y);
+            out[i]         = y;
+            if (a.aux && i > 0u) out[2u * N - i] = make_float2(y.x, -y.y);
+            if (i == 0u) out[N] = make_float2(zi[c].x - zi[c].y, 0.0f);
+         }
+      }
+   }
+}
+
+// Store stage of a group: every work item takes part (R2C with FPB > 1), or
+// work item (fl, t) writes transform first + fl.
+template <int N, int KIND, bool C>
+__device__ __forceinline__ void group_store(const float2 *smem, const float2 *data, uint32_t first, uint32_t nf,
+                                            uint32_t fl, uint32_t t, const FixedArgs &a,
+                                            const StoreMap<N, KIND, C> &map)
+{
+   if constexpr (KIND == KIND_R2C && r2c_group_store<N>()) r2c_store_group<N>(smem, first, nf, a);
+   else if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+}
+
 // Staging read of a group into LDS: work item threadIdx.x takes points
 // o = threadIdx.x + 256*c of the group (transform o / N, point o % N, so the
 // reads of each transform are coalesced). When a work item has at most 16
@@ -515,6 +585,81 @@ __device__ __forceinline__ void stage_group(const FixedArgs &a, uint32_t first, 
          const uint32_t k = i - f * (uint32_t) N;
          const float2   z = fixed_input<N, KIND>(a.x + in_base(a, first + f), k, a);
          smem[f * STRIDE + fslot(k)] = make_float2(sc.sx * z.x, sc.sy * z.y);
+      }
+   }
+}
+
+// C2R inMode 0 pre-split of a whole group when a workgroup holds FPB > 1
+// transforms: work item w takes the mirror pairs (i, N - i), i = w + 256c <
+// N/2, of every transform of the group, loading the four split factors of a
+// pair once (pair 0 takes those of N/2 for the self-mirrored Z[N/2], its
+// partner being the Nyquist bin). Same formula per point as the per-transform
+// pre-split in fixed_kernel. Used where every work item has pairs (N/2 >= 256)
+// and a group holds 2-4 transforms: C2R 2048 (503 MB per call) 244.0 ->
+// 220.4 us, 960 248.6 -> 217.3 us, 1024 239.9 -> 232.4 us; at 512 (8 per
+// group) it was slower (226.7 -> 249.1 us).
+template <int N>
+constexpr bool c2r_group_presplit()
+{
+   return Fixed<N>::P.FPB > 1 && Fixed<N>::P.FPB <= 4 && N / 2 >= kFixedThreads;
+}
+
+__device__ __forceinline__ float2 c2r_presplit(float2 x, float2 xm, float2 A, float2 B)
+{
+   return make_float2(x.x * A.x - x.y * A.y + (xm.x * B.x + xm.y * B.y),
+                      x.x * A.y + x.y * A.x + (xm.x * B.y - xm.y * B.x));
+}
+
+template <int N>
+__device__ __forceinline__ void c2r_presplit_group(const FixedArgs &a, uint32_t first, uint32_t nf, InScale sc,
+                                                   float2 *smem)
+{
+   constexpr int      FPB    = Fixed<N>::P.FPB;
+   constexpr uint32_t STRIDE = fixed_stride<N>();
+   constexpr uint32_t H      = (uint32_t) N / 2u;
+   constexpr int      GP     = (int) ((H + kFixedThreads - 1) / kFixedThreads);
+   constexpr int      GCH    = largest_divisor_upto(GP, 4);
+#pragma unroll 1
+   for (int c0 = 0; c0 < GP; c0 += GCH) {
+      float2 Ai[GCH], Bi[GCH], Aj[GCH], Bj[GCH];
+#pragma unroll
+      for (int c = 0; c < GCH; ++c) {
+         // pairs past N/2 (when 256 does not divide N/2) re-read pair N/2 - 1 and store nothing
+         const uint32_t i  = min(threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads), H - 1u);
+         const uint32_t j  = i == 0u ? H : (uint32_t) N - i;
+         Ai[c]             = split_factor(a, i);
+         Bi[c]             = split_factor(a, (uint32_t) N + i);
+         Aj[c]             = split_factor(a, j);
+         Bj[c]             = split_factor(a, (uint32_t) N + j);
+      }
+#pragma unroll
+      for (int f = 0; f < FPB; ++f) {
+         if ((uint32_t) f >= nf) break;
+         const float2 *in  = a.x + in_base(a, first + (uint32_t) f);
+         float2       *dst = smem + f * STRIDE;
+         float2        xi[GCH], xm[GCH];
+#pragma unroll
+         for (int c = 0; c < GCH; ++c) {
+            const uint32_t i = min(threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads), H - 1u);
+            xi[c]            = in[i];
+            xm[c]            = in[(uint32_t) N - i];   // i = 0: the Nyquist bin
+         }
+#pragma unroll
+         for (int c = 0; c < GCH; ++c) {
+            const uint32_t i = threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads);
+            if (H % kFixedThreads != 0 && i >= H) break;
+            const float2 zi = c2r_presplit(xi[c], xm[c], Ai[c], Bi[c]);
+            dst[fslot(i)]   = make_float2(sc.sx * zi.x, sc.sy * zi.y);
+            if (i != 0u) {
+               const float2 zj          = c2r_presplit(xm[c], xi[c], Aj[c], Bj[c]);
+               dst[fslot((uint32_t) N - i)] = make_float2(sc.sx * zj.x, sc.sy * zj.y);
+            }
+            else {
+               const float2 xh = in[H];
+               const float2 zh = c2r_presplit(xh, xh, Aj[c], Bj[c]);
+               dst[fslot(H)]   = make_float2(sc.sx * zh.x, sc.sy * zh.y);
+            }
+         }
       }
    }
 }
@@ -585,7 +730,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          if (gn < a.numGroups) pass0_load<N, KIND>(v, gn * FPB, fl, min((uint32_t) FPB, a.numChannels - gn * FPB), t, a);
          __syncthreads();
          fixed_passes_from<N, 1>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
          __syncthreads();
       }
    }
@@ -612,7 +757,10 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
-         if (fl < nf && a.aux == 0u) {
+         if (c2r_group_presplit<N>() && a.aux == 0u) {
+            c2r_presplit_group<N>(a, first, nf, sc, smem);
+         }
+         else if (fl < nf && a.aux == 0u) {
             const float2 *in = a.x + in_base(a, first + fl);
 #pragma unroll 1
             for (int c0 = 0; c0 < HALF; c0 += PCH) {
@@ -659,7 +807,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          }
          __syncthreads();
          fixed_passes_from<N, 0>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
          __syncthreads();
       }
    }
@@ -680,7 +828,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          }
          __syncthreads();
          fixed_passes_from<N, 1>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
          __syncthreads();
       }
    }
@@ -694,7 +842,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
          stage_group<N, KIND, STRIDE>(a, first, nf, sc, smem);
          __syncthreads();
          fixed_passes_from<N, 0>(data, t, twp);
-         if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
          __syncthreads();
       }
    }
@@ -1159,7 +1307,7 @@ constexpr uint32_t fixed_half_groups_per_wg()
       }
    }
    else if constexpr (KIND == KIND_R2C) {
-      return N == 32 ? 2u : N == 480 ? 4u : 0u;
+      return N == 32 ? 2u : N == 480 ? 4u : N == 512 ? 6u : N == 1024 ? 8u : 0u;
    }
    else {
       switch (N) {
