@@ -540,13 +540,15 @@ __device__ __forceinline__ void r2c_store_group(const float2 *smem, uint32_t fir
    }
 }
 
-// C2C store of a group whose outputs are contiguous (no output offsets,
-// stride N) for lengths with at most 8 work items per transform: work item w
+// C2C / C2R store of a group whose outputs are contiguous (no output offsets,
+// stride N complex points) for lengths with at most 8 work items per
+// transform (the C2R output is the inner complex FFT's): work item w
 // writes points o = w + 256c of the group's FPB * N outputs, so every store
 // instruction covers consecutive addresses (per transform, T work items would
 // otherwise write runs of T points, up to 64 lines per store instruction).
 // 503 MB per call: N = 16 1243 -> 222 us, 32 1113 -> 224 us, 64 315 -> 226
-// us, 128 254 -> 229 us, 48 326 -> 247 us, 96 274 -> 250 us.
+// us, 128 254 -> 229 us, 48 326 -> 247 us, 96 274 -> 250 us; C2R 32 1030 ->
+// 318 us, 64 398 -> 321 us, 128 353 -> 316 us.
 template <int N>
 constexpr bool c2c_linear_store()
 {
@@ -582,7 +584,7 @@ __device__ __forceinline__ void c2c_store_linear(const float2 *smem, uint32_t fi
 }
 
 // Store stage of a group: every work item takes part (R2C with FPB > 1, C2C
-// with contiguous outputs of lengths up to 128), or work item (fl, t) writes
+// and C2R with contiguous outputs of lengths up to 128), or work item (fl, t) writes
 // transform first + fl.
 template <int N, int KIND, bool C>
 __device__ __forceinline__ void group_store(const float2 *smem, const float2 *data, uint32_t first, uint32_t nf,
@@ -590,7 +592,7 @@ __device__ __forceinline__ void group_store(const float2 *smem, const float2 *da
                                             const StoreMap<N, KIND, C> &map)
 {
    if constexpr (KIND == KIND_R2C && r2c_group_store<N>()) r2c_store_group<N>(smem, first, nf, a);
-   else if constexpr (KIND == KIND_C2C && c2c_linear_store<N>()) {
+   else if constexpr ((KIND == KIND_C2C || KIND == KIND_C2R) && c2c_linear_store<N>()) {
       if (a.outOff == nullptr && a.outStride == (uint32_t) N) c2c_store_linear<N>(smem, first, nf, a);
       else if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
    }
@@ -1356,8 +1358,7 @@ constexpr uint32_t fixed_half_groups_per_wg()
    }
    else {
       switch (N) {
-      case 128: return 2;
-      case 32: case 64: case 256: case 480: case 512: return 3;
+      case 256: case 480: case 512: return 3;
       default: return 0;
       }
    }
