@@ -583,6 +583,63 @@ __device__ __forceinline__ void c2c_store_linear(const float2 *smem, uint32_t fi
    }
 }
 
+// R2C (outmode 0) store of a group whose output rows of N + 1 bins are
+// contiguous, for lengths with at most 2 work items per transform (N = 32):
+// work item w writes bins o = w + 256c of the group's FPB (N + 1) outputs
+// (transform o / (N + 1), bin o % (N + 1)), so every store instruction
+// covers consecutive addresses. Same formula per bin as fixed_store,
+// including bin 0 and the Nyquist bin N. 503 MB per call: N = 32 447.5 ->
+// 354.0 us; at N = 64 and 128 it raised register use past 3 workgroups per
+// CU and was 4-8% slower, so they keep the per-transform store.
+template <int N>
+constexpr bool r2c_linear_store()
+{
+   return Fixed<N>::P.T <= 2 && !r2c_group_store<N>();
+}
+
+template <int N>
+__device__ __forceinline__ void r2c_store_linear(const float2 *smem, uint32_t first, uint32_t nf, const FixedArgs &a)
+{
+   constexpr int      FPB    = Fixed<N>::P.FPB;
+   constexpr uint32_t STRIDE = fixed_stride<N>();
+   constexpr uint32_t ROW    = (uint32_t) N + 1u;
+   constexpr int      IT     = (int) ((FPB * ROW + kFixedThreads - 1) / kFixedThreads);
+   constexpr int      CH     = 8;
+   float2            *out    = a.y + (size_t) first * ROW;
+   const uint32_t     total  = nf * ROW;
+#pragma unroll 1
+   for (int c0 = 0; c0 < IT; c0 += CH) {
+      float2 zi[CH], zm[CH], A[CH], B[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+         // bins past the group (or past IT chunks) re-read bin 0 of transform 0 and store nothing
+         uint32_t o = threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads);
+         if (c0 + c >= IT || o >= total) o = 0u;
+         const uint32_t f  = o / ROW, i = o - f * ROW;
+         const uint32_t ii = i < (uint32_t) N ? i : 0u;   // the Nyquist bin reads Z[0]
+         const float2  *src = smem + f * STRIDE;
+         zi[c]              = src[fslot(fixed_position<N>(ii))];
+         zm[c]              = src[fslot(fixed_position<N>(ii == 0u ? 0u : (uint32_t) N - ii))];
+         A[c]               = split_factor(a, ii);
+         B[c]               = split_factor(a, 2u * N - 1u - ii);
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+         const uint32_t o = threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads);
+         if (c0 + c >= IT || o >= total) break;
+         const uint32_t f = o / ROW, i = o - f * ROW;
+         if (i == (uint32_t) N) {
+            out[o] = make_float2(zi[c].x - zi[c].y, 0.0f);
+            continue;
+         }
+         float2 t1;
+         if (i == 0u) t1 = make_float2(zi[c].x * B[c].x + zi[c].y * B[c].y, zi[c].x * B[c].x - zi[c].y * B[c].y);
+         else t1 = make_float2(zm[c].x * B[c].x + zm[c].y * B[c].y, zm[c].x * B[c].y - zm[c].y * B[c].x);
+         out[o] = make_float2(zi[c].x * A[c].x - zi[c].y * A[c].y + t1.x, zi[c].x * A[c].y + zi[c].y * A[c].x + t1.y);
+      }
+   }
+}
+
 // Store stage of a group: every work item takes part (R2C with FPB > 1, C2C
 // and C2R with contiguous outputs of lengths up to 128), or work item (fl, t) writes
 // transform first + fl.
@@ -592,6 +649,11 @@ __device__ __forceinline__ void group_store(const float2 *smem, const float2 *da
                                             const StoreMap<N, KIND, C> &map)
 {
    if constexpr (KIND == KIND_R2C && r2c_group_store<N>()) r2c_store_group<N>(smem, first, nf, a);
+   else if constexpr (KIND == KIND_R2C && r2c_linear_store<N>()) {
+      if (a.aux == 0u && a.outOff == nullptr && a.outStride == (uint32_t) N + 1u)
+         r2c_store_linear<N>(smem, first, nf, a);
+      else if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
+   }
    else if constexpr ((KIND == KIND_C2C || KIND == KIND_C2R) && c2c_linear_store<N>()) {
       if (a.outOff == nullptr && a.outStride == (uint32_t) N) c2c_store_linear<N>(smem, first, nf, a);
       else if (fl < nf) fixed_store<N, KIND>(data, first + fl, t, a, map);
