@@ -62,6 +62,8 @@ def parse():
                     help="after the W warmup steps, keep stepping (untimed) until this much wall time has passed: "
                          "the GPU clocks ramp up over tens of ms of sustained work after the idle host-side setup")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="c2c_960 fp32: skip timing the split-precision MFMA variants after the headline")
     ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target seconds per CPU baseline run")
     return ap.parse_args()
 
@@ -162,7 +164,8 @@ class Workload:
         res = {"value": flops / t / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
                "sample": f"oracle FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_cn, N=960, {cnt} of 65536 channels, "
                          f"median of 5 runs ({t:.3f} s each), input re-copied before each run"}
-        threads = min(os.cpu_count() or 1, 16)
+        threads, cores_note = usable_cores()
+        res["cpu_model"] = cpu_model()
         if threads > 1:
             tt = []
             for _ in range(3):
@@ -172,7 +175,8 @@ class Workload:
                 oracle.raw_threads_c2c(xin, w, y, n, cnt, False, threads)
                 tt.append(time.perf_counter() - t0)
             res["all_cores"] = {"value": flops / float(np.median(tt)) / 1e9, "cores": threads,
-                                "sample": "same sample, channels split over POSIX threads"}
+                                "cores_note": cores_note,
+                                "sample": "same sample, channels split over POSIX threads, median of 3"}
         return res
 
     # -- the N=960 workload with host-resident buffers (PCIe-inclusive) --------
@@ -502,6 +506,34 @@ class Workload:
                 "sample": f"oracle R2C+C2R _cn, N=4096, {cnt} of 16384 channels, median of 5 ({tm:.3f} s)"}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def usable_cores():
+    """Threads for the all-cores CPU baseline: the CPUs this process may run
+    on (affinity mask), bounded by the cgroup CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    note = f"affinity mask {n} CPUs, os.cpu_count() {os.cpu_count()}"
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            quota, period = fh.read().split()[:2]
+        if quota != "max":
+            q = max(1, int(math.ceil(int(quota) / int(period))))
+            note += f", cgroup quota {q} CPUs"
+            n = min(n, q)
+    except (OSError, ValueError):
+        pass
+    return max(1, n), note
+
+
 def torch_equal(a, b):
     return bool((a == b).all().item())
 
@@ -516,6 +548,96 @@ def load_traffic(workload, kernel):
         if t.get("kernel") == kernel:
             return t
     return None
+
+
+def time_steps(wl, args, stream, min_warmup_s):
+    """W warmup steps, untimed steps until min_warmup_s of wall time, then K
+    timed steps between HIP events on the launch stream. Returns (ms per
+    step, wall seconds of the timed steps, ramp steps, ramp seconds)."""
+    import torch
+
+    def settle():
+        if hasattr(wl, "finish"):
+            wl.finish()
+        torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        wl.step()
+    settle()
+    # Clock ramp: after the host-side setup the device idles at low clocks and
+    # needs tens of ms of sustained work to reach steady state (measured on
+    # MI355X for the N=960 call: 385 us per launch for the first 10, 176 us
+    # after about 100). Steps run in chunks that double until one takes 20 ms.
+    ramp_steps, chunk, t_ramp = 0, 1, time.perf_counter()
+    while time.perf_counter() - t_ramp < min_warmup_s:
+        t_chunk = time.perf_counter()
+        for _ in range(chunk):
+            wl.step()
+        settle()
+        ramp_steps += chunk
+        if time.perf_counter() - t_chunk < 0.02:
+            chunk *= 2
+    ramp_s = time.perf_counter() - t_ramp
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        wl.step()
+    if hasattr(wl, "finish"):
+        wl.finish()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1) / args.steps
+    if getattr(wl, "wall_timing", False):
+        ms = wall * 1e3 / args.steps       # host-to-host: the pipeline's own streams
+    return ms, wall, ramp_steps, ramp_s
+
+
+# Error bars of the C2C variants against the FP64 DFT: the split-precision
+# goal is 2e-6; BF16x2 (three bf16 products) is reported with its own bar.
+VARIANT_BARS = {"fp32": 1e-6, "bf16x3": 2e-6, "fp16x2": 2e-6, "bf16x2": 1e-5}
+VARIANT_CODES = {"fp32": 0, "bf16x3": 1, "fp16x2": 2, "bf16x2": 3}
+
+
+def check_passes(args, chk):
+    if args.workload in ("c2c_960", "c2c_960_host"):
+        if args.variant == "fp32":
+            return chk["rel_l2_vs_oracle"] <= 1e-6 and chk["rel_l2_vs_fp64"] <= 1e-6
+        return chk["rel_l2_vs_fp64"] <= VARIANT_BARS[args.variant]
+    if args.workload == "ragged_10000":
+        return chk["fwd_rel_l2_vs_fp64"] <= 1e-6 and chk["roundtrip_rel_l2"] <= 1e-6
+    if args.workload == "r2c_4096":
+        return chk["fwd_rel_l2_vs_fp64"] <= 1e-6 and chk["roundtrip_rel_l2"] <= 1e-6
+    if args.workload == "i16_256":
+        return bool(chk["bit_exact_vs_oracle"])
+    if args.workload == "copy_503m":
+        return bool(chk["copied_equal"])
+    return True
+
+
+def time_variants(wl, args, lib, F, torch, stream, fp32_ms):
+    """The split-precision MFMA variants of the same call, timed in the same
+    process right after the fp32 headline: ms per step, roofline fraction,
+    relative L2 against the FP64 DFT on 256 channels, and the ratio to the
+    fp32 time (below 1 = faster than fp32)."""
+    res = {}
+    try:
+        for name in ("bf16x3", "fp16x2", "bf16x2"):
+            assert lib.FFTLIB_GPU_setC2CVariant(VARIANT_CODES[name]) == 0, F.last_error()
+            lib.FFTLIB_GPU_setSynchronous(0)
+            ms, _, _, _ = time_steps(wl, args, stream, 0.15)
+            lib.FFTLIB_GPU_setSynchronous(1)
+            chk = wl.check_c2c_960()
+            frac = wl.bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
+            res[name] = {"ms_per_step": round(ms, 5), "gflops": round(wl.flops / (ms / 1e3) / 1e9, 1),
+                         "roofline_frac": round(frac, 4), "rel_l2_vs_fp64": chk["rel_l2_vs_fp64"],
+                         "bar": VARIANT_BARS[name], "within_bar": chk["rel_l2_vs_fp64"] <= VARIANT_BARS[name],
+                         "time_vs_fp32": round(ms / fp32_ms, 4),
+                         "kernel": "c2c960_mfma_kernel<%s>" % name}
+    finally:
+        assert lib.FFTLIB_GPU_setC2CVariant(0) == 0
+    return res
 
 
 def main():
@@ -542,43 +664,8 @@ def main():
 
     wl = Workload(args, F, lib, torch)
     lib.FFTLIB_GPU_setSynchronous(0)
-
-    def settle():
-        if hasattr(wl, "finish"):
-            wl.finish()
-        torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        wl.step()
-    settle()
-    # Clock ramp: after the host-side setup the device idles at low clocks and
-    # needs tens of ms of sustained work to reach steady state (measured on
-    # MI355X for the N=960 call: 385 us per launch for the first 10, 176 us
-    # after about 100). Steps run in chunks that double until one takes 20 ms.
-    ramp_steps, chunk, t_ramp = 0, 1, time.perf_counter()
-    while time.perf_counter() - t_ramp < args.min_warmup_s:
-        t_chunk = time.perf_counter()
-        for _ in range(chunk):
-            wl.step()
-        settle()
-        ramp_steps += chunk
-        if time.perf_counter() - t_chunk < 0.02:
-            chunk *= 2
-    ramp_s = time.perf_counter() - t_ramp
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        wl.step()
-    if hasattr(wl, "finish"):
-        wl.finish()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    ms, wall, ramp_steps, ramp_s = time_steps(wl, args, stream, args.min_warmup_s)
     lib.FFTLIB_GPU_setSynchronous(1)
-    ms = ev0.elapsed_time(ev1) / args.steps
-    if getattr(wl, "wall_timing", False):
-        ms = wall * 1e3 / args.steps       # host-to-host: the pipeline's own streams
     t = ms / 1e3
     achieved = wl.bytes / t / 1e9
     value = wl.flops / t / 1e9 if wl.flops else achieved
@@ -607,9 +694,17 @@ def main():
     if hasattr(wl, "extra_" + args.workload):
         out.update(getattr(wl, "extra_" + args.workload)(max(3, args.steps // 4)))
     out["check"] = getattr(wl, "check_" + args.workload)()
+    ok = check_passes(args, out["check"])
+    out["check"]["passed"] = ok
+    if args.workload == "c2c_960" and args.variant == "fp32" and not args.generic and not args.no_variants:
+        out["variants"] = time_variants(wl, args, lib, F, torch, stream, ms)
+        ok = ok and all(v["within_bar"] for v in out["variants"].values() if v["bar"] <= 2e-6)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = getattr(wl, "cpu_" + args.workload)(args.cpu_seconds)
     print(json.dumps(out), flush=True)
+    if not ok:
+        print("bench: result check failed (see the check / variants fields)", file=sys.stderr, flush=True)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -61,9 +61,21 @@ def _compile(src, verbose):
     return obj
 
 
+def clean_stale():
+    """Removes objects (and interrupted temporaries) in _build/ whose source is
+    no longer in SOURCES, so they are neither linked nor shipped."""
+    if not os.path.isdir(OBJ_DIR):
+        return
+    keep = {s + ".o" for s in SOURCES}
+    for f in os.listdir(OBJ_DIR):
+        if f not in keep:
+            os.remove(os.path.join(OBJ_DIR, f))
+
+
 def build(verbose=False):
     """Compile all sources and link libfftlib_amd.so; returns the library path."""
     os.makedirs(OBJ_DIR, exist_ok=True)
+    clean_stale()
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))

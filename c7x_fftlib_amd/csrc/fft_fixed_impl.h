@@ -561,7 +561,10 @@ __device__ __forceinline__ void c2c_store_linear(const float2 *smem, uint32_t fi
    constexpr int      FPB    = Fixed<N>::P.FPB;
    constexpr uint32_t STRIDE = fixed_stride<N>();
    constexpr int      IT     = FPB * N / kFixedThreads;
-   constexpr int      CH     = IT < 8 ? IT : 8;
+   // chunks divide IT, so every LDS read below stays inside the group's
+   // FPB transforms (N = 48 and 96 have IT = 12: chunks of 6)
+   constexpr int      CH     = largest_divisor_upto(IT, 8);
+   static_assert(IT % CH == 0 && FPB * N % kFixedThreads == 0, "linear store chunks must tile the group");
    float2            *out    = a.y + (size_t) first * N;
    const float        sy     = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
    const bool         pdiv   = a.mode == C2C_INVERSE_POSTDIV;

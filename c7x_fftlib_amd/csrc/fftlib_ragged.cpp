@@ -159,6 +159,7 @@ void release(FFTLIB_GPU_ragged_s *h)
 hipError_t fork_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
 {
    hipError_t e;
+   if (ev >= h->events.size()) return hipErrorInvalidValue;   // pool sized by ragged_event_demand
    hipEvent_t fork = h->events[ev++];
    if ((e = hipEventRecord(fork, origin)) != hipSuccess) return e;
    for (hipStream_t lane : h->lanes)
@@ -169,6 +170,7 @@ hipError_t join_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
 {
    hipError_t e;
    for (hipStream_t lane : h->lanes) {
+      if (ev >= h->events.size()) return hipErrorInvalidValue;
       hipEvent_t join = h->events[ev++];
       if ((e = hipEventRecord(join, lane)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(origin, join, 0)) != hipSuccess) return e;
