@@ -626,7 +626,7 @@ def time_variants(wl, args, lib, F, torch, stream, fp32_ms):
         for name in ("bf16x3", "fp16x2", "bf16x2"):
             assert lib.FFTLIB_GPU_setC2CVariant(VARIANT_CODES[name]) == 0, F.last_error()
             lib.FFTLIB_GPU_setSynchronous(0)
-            ms, _, _, _ = time_steps(wl, args, stream, 0.15)
+            ms, _, _, _ = time_steps(wl, args, stream, min(0.15, args.min_warmup_s))
             lib.FFTLIB_GPU_setSynchronous(1)
             chk = wl.check_c2c_960()
             frac = wl.bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
