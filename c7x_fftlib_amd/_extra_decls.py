@@ -9,6 +9,7 @@ _U32 = ctypes.c_uint32
 # exports them.
 DECLS = {
     "FFTLIB_GPU_kernelAsync": (_S, [_P, _P, _P, _P, _P, _P]),
+    "FFTLIB_GPU_kernelAsyncNoAlloc": (_S, [_P, _P, _P, _P, _P, _P]),
     "FFTLIB_GPU_pipelineCreate": (_S, [_P, _U32]),
     "FFTLIB_GPU_pipelineSubmit": (_S, [_P, _P, _P, _P, _P, _P]),
     "FFTLIB_GPU_pipelineWait": (_S, [_P]),

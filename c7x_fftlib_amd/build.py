@@ -32,6 +32,7 @@ SOURCES = [
     "fftlib_ragged.cpp",
     "fftlib_mma.cpp",
     "fftlib_async.cpp",
+    "fftlib_noalloc.hip",
 ]
 
 ARCH = "gfx950"

@@ -347,6 +347,115 @@ ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const 
 
 uint32_t class_group_size(uint32_t n) { return fixed_fpb(n); }
 
+// ---- two-pass tables in caller memory (no allocation) ---------------------------
+namespace {
+
+// One block of the reference twiddle layout (FFTLIB_tw_gen_c2c's loops): the
+// radix-r stage with stride j of the radix group that starts at length ng,
+// t = ng / (r j) butterflies; entry (i', m) of the block, i' < t, m = 1..r-1,
+// is W^(m i' j) over ng at complex position k0 + (m-1) t + i'.
+struct TwBlock {
+   uint32_t r, ng, j, t, k0, first;   // first: index of the block's first thread
+};
+constexpr int kMaxTwBlocks = 48;       // two tables of at most 2 + 2 + 11 stages each
+struct TwGenArgs {
+   float2  *dst;
+   uint32_t nBlocks;
+   uint32_t twEntries;                 // threads of the stage tables (both lengths)
+   uint32_t n, L, nh, colOff;          // column tables: W_n^e (e < L) at colOff, W_n^(hL) (h < nh) after
+   TwBlock  b[kMaxTwBlocks];
+};
+
+// Blocks of length n's stage table at complex offset base; same loop bounds
+// and index arithmetic as FFTLIB_tw_gen_c2c.
+bool tw_blocks(int32_t numPoints, uint32_t base, TwGenArgs &a)
+{
+   const int32_t rads[3] = {5, 3, 4};
+   int32_t       k = 0, ng = numPoints;
+   for (int32_t g = 0; g < 3; ++g) {
+      const int32_t r = rads[g];
+      if (ng <= 0 || ng % r != 0) continue;
+      int32_t len = ng, t = ng / r;
+      for (int32_t j = 1; j <= ng / r && len % r == 0; j *= r) {
+         if (a.nBlocks >= (uint32_t) kMaxTwBlocks) return false;
+         a.b[a.nBlocks++] = TwBlock{(uint32_t) r, (uint32_t) ng, (uint32_t) j, (uint32_t) t, base + (uint32_t) k / 2u,
+                                    a.twEntries};
+         a.twEntries += (uint32_t) t * (uint32_t) (r - 1);
+         k += 2 * t * (r - 1);
+         t /= r;
+         len /= r;
+      }
+      if (r != 4) ng = len;
+   }
+   return true;
+}
+
+// One work item per table entry: the stage tables of both lengths in the
+// host generator's arithmetic (PI = 3.141592654, angle 2 m pi i / ng in
+// double), then the column tables (angle 2 pi e / n with the double-precision
+// pi, as two_pass_plan builds them).
+__global__ void __launch_bounds__(256) two_pass_tables_kernel(TwGenArgs a)
+{
+   const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+   if (e < a.twEntries) {
+      uint32_t bi = 0;
+      while (bi + 1u < a.nBlocks && a.b[bi + 1u].first <= e) ++bi;
+      const TwBlock  b  = a.b[bi];
+      const uint32_t q  = e - b.first;
+      const uint32_t ip = q / (b.r - 1u), m = 1u + q % (b.r - 1u);
+      const double   pi = 3.141592654;
+      const double   an = 2.0 * (double) m * pi * (double) (ip * b.j) / (double) b.ng;
+      a.dst[b.k0 + (m - 1u) * b.t + ip] = make_float2((float) cos(an), (float) -sin(an));
+      return;
+   }
+   const uint32_t c = e - a.twEntries;
+   if (c < a.L + a.nh) {
+      const uint64_t ex = c < a.L ? (uint64_t) c : (uint64_t) (c - a.L) * a.L;
+      const double   an = 2.0 * M_PI * (double) ex / (double) a.n;
+      a.dst[a.colOff + c] = make_float2((float) cos(an), (float) -sin(an));
+   }
+}
+
+void col_table_sizes(uint32_t n, uint32_t *bits, uint32_t *L, uint32_t *nh)
+{
+   uint32_t b = 0;
+   while ((1ull << (2u * b)) < n) ++b;
+   *bits = b;
+   *L    = 1u << b;
+   *nh   = (n + *L - 1u) / *L;
+}
+
+}  // namespace
+
+size_t two_pass_table_bytes(uint32_t n)
+{
+   uint32_t n1 = 0, m2 = 0, bits, L, nh;
+   if (!two_pass_split(n, &n1, &m2)) return 0;
+   col_table_sizes(n, &bits, &L, &nh);
+   return ((size_t) n1 + m2 + L + nh) * sizeof(float2);
+}
+
+hipError_t two_pass_plan_at(uint32_t n, float *dst, hipStream_t stream, TwoPassPlan *plan)
+{
+   uint32_t n1 = 0, m2 = 0, bits, L, nh;
+   if (!two_pass_split(n, &n1, &m2)) return hipErrorInvalidValue;
+   col_table_sizes(n, &bits, &L, &nh);
+   TwGenArgs a{};
+   a.dst = (float2 *) dst;
+   if (!tw_blocks((int32_t) n1, 0u, a) || !tw_blocks((int32_t) m2, n1, a)) return hipErrorInvalidValue;
+   a.n = n, a.L = L, a.nh = nh, a.colOff = n1 + m2;
+   const uint32_t threads = a.twEntries + L + nh;
+   hipLaunchKernelGGL(two_pass_tables_kernel, dim3((threads + 255u) / 256u), dim3(256), 0, stream, a);
+   hipError_t e = hipGetLastError();
+   if (e != hipSuccess) return e;
+   TwoPassPlan p;
+   p.n = n, p.n1 = n1, p.m2 = m2, p.s2 = (m2 + 15u) & ~15u;
+   p.tw1 = dst, p.tw2 = dst + 2u * n1, p.colTw = dst + 2u * ((size_t) n1 + m2);
+   p.colTwHi = p.colTw + 2u * L, p.colBits = bits;
+   *plan = p;
+   return hipSuccess;
+}
+
 hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, float *scratch,
                                hipStream_t stream)
 {
@@ -354,6 +463,14 @@ hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t nu
    TwoPassPlan p;
    hipError_t  e = two_pass_plan(n, &p);
    if (e != hipSuccess) return e;
+   return launch_c2c_two_pass_plan(p, x, y, numChannels, mode, scratch, stream);
+}
+
+hipError_t launch_c2c_two_pass_plan(const TwoPassPlan &p, const float *x, float *y, uint32_t numChannels, int mode,
+                                    float *scratch, hipStream_t stream)
+{
+   if (numChannels == 0) return hipSuccess;
+   hipError_t e;
    if ((uint64_t) numChannels * p.s2 >= (1ull << 32) || (uint64_t) numChannels * p.n1 >= (1ull << 32))
       return hipErrorInvalidValue;
    ClassArgs c{};

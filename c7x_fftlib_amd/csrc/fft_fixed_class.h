@@ -72,4 +72,16 @@ fixedk::ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count
 hipError_t launch_c2c_two_pass(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, float *scratch,
                                hipStream_t stream);
 
+// The same with the tables of a given plan.
+hipError_t launch_c2c_two_pass_plan(const TwoPassPlan &p, const float *x, float *y, uint32_t numChannels, int mode,
+                                    float *scratch, hipStream_t stream);
+
+// Bytes of n's two-pass tables (0 when n has no two-pass split), and a plan
+// whose tables a kernel enqueued on `stream` writes to device memory at dst
+// (two_pass_table_bytes(n) bytes, 8-byte aligned): nothing is allocated. The
+// values are those two_pass_plan computes on the host (same formulas in
+// double precision).
+size_t     two_pass_table_bytes(uint32_t n);
+hipError_t two_pass_plan_at(uint32_t n, float *dst, hipStream_t stream, TwoPassPlan *plan);
+
 }  // namespace fftlib

@@ -30,16 +30,6 @@ FFTLIB_STATUS fail(const char *what, hipError_t err)
    return FFTLIB_ERR_FAILURE;
 }
 
-// Plan of pBlock as _init wrote it (any of the ten kernels).
-FFTLIB_STATUS load_plan(const void *pBlock, Plan *p)
-{
-   if (pBlock == nullptr) return FFTLIB_ERR_NULL_POINTER;
-   memcpy(p, pBlock, sizeof *p);
-   if (p->magic != kPlanMagic || p->kernel < K_C2C_BATCHED || p->kernel > K_C2R_SINGLE) return FFTLIB_ERR_INVALID_VALUE;
-   if (p->style == FFTLIB_FUNCTION_NATC) return FFTLIB_ERR_NOT_IMPLEMENTED;
-   return FFTLIB_SUCCESS;
-}
-
 size_t round_up(size_t v, size_t a) { return (v + a - 1u) / a * a; }
 
 // True for memory the GPU's copy engines can address directly: pinned host
@@ -100,6 +90,15 @@ hipError_t finish(Slot &s)
 }
 
 }  // namespace
+
+FFTLIB_STATUS fftlib::load_plan(const void *pBlock, Plan *p)
+{
+   if (pBlock == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   memcpy(p, pBlock, sizeof *p);
+   if (p->magic != kPlanMagic || p->kernel < K_C2C_BATCHED || p->kernel > K_C2R_SINGLE) return FFTLIB_ERR_INVALID_VALUE;
+   if (p->style == FFTLIB_FUNCTION_NATC) return FFTLIB_ERR_NOT_IMPLEMENTED;
+   return FFTLIB_SUCCESS;
+}
 
 struct FFTLIB_GPU_Pipeline_s {
    int               device = 0;

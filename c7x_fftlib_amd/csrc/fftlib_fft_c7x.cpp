@@ -8,6 +8,7 @@
 // buffers instead of faulting, and pX is never written (the reference uses it
 // as scratch). The real transforms do not write pXFFT (half lengths that do
 // not fit one workgroup's LDS go through library scratch instead).
+#include <stdint.h>
 #include <string.h>
 
 #include "fftlib.h"
@@ -80,31 +81,73 @@ FFTLIB_STATUS read_plan(const void *pBlock, KernelId kernel, uint32_t numPoints,
 }
 
 // ---- launchers bound to plans ---------------------------------------------
-// Batched complex FFT of st.n points: the MFMA kernel when selected, the
-// dedicated kernel of the length, two passes on the multi-length kernels for
-// long lengths, else the generic LDS or outer-pass kernels.
-hipError_t run_complex(const StageTable &st, const float *x, float *y, const float *tw, uint32_t numChannels,
-                       int mode, hipStream_t s)
+// Routes of a batched complex FFT of st.n points: the MFMA kernel when
+// selected, the specialised (dedicated or class) kernel of the length, two
+// passes on the class kernels for long lengths, else the generic LDS or
+// outer-pass kernels.
+enum ComplexRoute { ROUTE_MFMA, ROUTE_FIXED, ROUTE_TWO_PASS, ROUTE_LARGE, ROUTE_LDS };
+
+ComplexRoute complex_route(const StageTable &st)
 {
-   const int variant = c2c_variant();
-   if (variant != C2C_VARIANT_FP32 && mfma_c2c_supported(st.n))
-      return launch_c2c_mfma(x, y, st.n, numChannels, mode, variant, s);
-   if (specialized_kernels() && fixed_c2c_supported(st.n))
-      return launch_c2c_fixed(x, y, tw, st.n, numChannels, mode, s);
    uint32_t n1 = 0, m2 = 0;
-   if (specialized_kernels() && two_pass_split(st.n, &n1, &m2)) {
-      void      *scratch = nullptr;
-      hipError_t e       = device_scratch(two_pass_scratch_points(st.n, numChannels) * 8u, s, SCRATCH_COMPLEX, &scratch);
-      if (e != hipSuccess) return e;
-      return launch_c2c_two_pass(x, y, st.n, numChannels, mode, (float *) scratch, s);
+   if (c2c_variant() != C2C_VARIANT_FP32 && mfma_c2c_supported(st.n)) return ROUTE_MFMA;
+   if (specialized_kernels() && fixed_c2c_supported(st.n)) return ROUTE_FIXED;
+   if (specialized_kernels() && two_pass_split(st.n, &n1, &m2)) return ROUTE_TWO_PASS;
+   if (!fits_lds(st)) return ROUTE_LARGE;
+   return ROUTE_LDS;
+}
+
+size_t aligned256(size_t v) { return (v + 255u) & ~(size_t) 255u; }
+
+// Workspace bytes of one complex FFT call (0 for the routes without scratch;
+// SIZE_MAX for the MFMA route, whose tables are built on first use).
+size_t complex_workspace(const StageTable &st, uint32_t numChannels)
+{
+   switch (complex_route(st)) {
+   case ROUTE_MFMA: return SIZE_MAX;
+   case ROUTE_TWO_PASS:
+      return aligned256(two_pass_table_bytes(st.n)) + aligned256(two_pass_scratch_points(st.n, numChannels) * 8u);
+   case ROUTE_LARGE: return aligned256((size_t) st.n * numChannels * 8u);
+   default: return 0;
    }
-   if (!fits_lds(st)) {
+}
+
+// Scratch of a launcher: from the call's workspace when it has one, else the
+// library's grow-only buffer of (device, stream, role).
+hipError_t call_scratch(Workspace *ws, size_t bytes, hipStream_t s, int role, void **ptr)
+{
+   if (ws) return ws->take(bytes, ptr);
+   return device_scratch(bytes, s, role, ptr);
+}
+
+hipError_t run_complex(const StageTable &st, const float *x, float *y, const float *tw, uint32_t numChannels,
+                       int mode, hipStream_t s, Workspace *ws)
+{
+   switch (complex_route(st)) {
+   case ROUTE_MFMA:
+      if (ws) return hipErrorNotSupported;   // device tables built on first use
+      return launch_c2c_mfma(x, y, st.n, numChannels, mode, c2c_variant(), s);
+   case ROUTE_FIXED: return launch_c2c_fixed(x, y, tw, st.n, numChannels, mode, s);
+   case ROUTE_TWO_PASS: {
       void      *scratch = nullptr;
-      hipError_t e       = device_scratch((size_t) st.n * numChannels * 8u, s, SCRATCH_COMPLEX, &scratch);
+      hipError_t e       = call_scratch(ws, two_pass_scratch_points(st.n, numChannels) * 8u, s, SCRATCH_COMPLEX,
+                                        &scratch);
+      if (e != hipSuccess) return e;
+      if (ws == nullptr) return launch_c2c_two_pass(x, y, st.n, numChannels, mode, (float *) scratch, s);
+      void       *tables = nullptr;
+      TwoPassPlan plan;
+      if ((e = ws->take(two_pass_table_bytes(st.n), &tables)) != hipSuccess) return e;
+      if ((e = two_pass_plan_at(st.n, (float *) tables, s, &plan)) != hipSuccess) return e;
+      return launch_c2c_two_pass_plan(plan, x, y, numChannels, mode, (float *) scratch, s);
+   }
+   case ROUTE_LARGE: {
+      void      *scratch = nullptr;
+      hipError_t e       = call_scratch(ws, (size_t) st.n * numChannels * 8u, s, SCRATCH_COMPLEX, &scratch);
       if (e != hipSuccess) return e;
       return launch_c2c_large(x, y, tw, st, numChannels, mode, (float *) scratch, s);
    }
-   return launch_c2c(x, y, tw, st, numChannels, mode, s);
+   default: return launch_c2c(x, y, tw, st, numChannels, mode, s);
+   }
 }
 
 hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
@@ -112,7 +155,7 @@ hipError_t run_c2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
    int mode = C2C_FORWARD;
    if (p.kernel == K_IC2C_BATCHED) mode = C2C_INVERSE_PRESCALE;
    if (p.kernel == K_IC2C_SINGLE) mode = C2C_INVERSE_POSTDIV;
-   return run_complex(p.st, (const float *) d.in, (float *) d.out, (const float *) d.tw, p.numChannels, mode, s);
+   return run_complex(p.st, (const float *) d.in, (float *) d.out, (const float *) d.tw, p.numChannels, mode, s, d.ws);
 }
 
 // Real transforms: the fused kernel of the half length (complex FFT in LDS
@@ -136,6 +179,13 @@ bool real_fused_generic(const Plan &p)
           (uint64_t) p.st.n * p.numChannels <= kRealFusedMaxPoints;
 }
 
+// True when a real transform runs as a complex pass plus a split pass
+// through the half-length spectrum buffer.
+bool real_two_launch(const Plan &p)
+{
+   return !(specialized_kernels() && fixed_real_supported(p.st.n)) && !real_fused_generic(p);
+}
+
 hipError_t run_r2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
 {
    const float *x = (const float *) d.in, *tw = (const float *) d.tw, *sf = (const float *) d.aux0;
@@ -144,8 +194,8 @@ hipError_t run_r2c(const Plan &p, const DeviceBuffers &d, hipStream_t s)
       return launch_r2c_fixed(x, y, tw, sf, p.st.n, p.numChannels, p.mode, s);
    if (real_fused_generic(p)) return launch_r2c(x, y, tw, sf, p.st, p.numChannels, p.mode, s);
    void      *z = nullptr;
-   hipError_t e = device_scratch((size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
-   if (e == hipSuccess) e = run_complex(p.st, x, (float *) z, tw, p.numChannels, C2C_FORWARD, s);
+   hipError_t e = call_scratch(d.ws, (size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
+   if (e == hipSuccess) e = run_complex(p.st, x, (float *) z, tw, p.numChannels, C2C_FORWARD, s, d.ws);
    if (e == hipSuccess) e = launch_r2c_split((const float *) z, y, sf, p.st.n, p.numChannels, p.mode, s);
    return e;
 }
@@ -159,11 +209,11 @@ hipError_t run_c2r(const Plan &p, const DeviceBuffers &d, hipStream_t s)
       return launch_c2r_fixed(x, y, tw, sf, p.st.n, p.numChannels, p.mode, postdiv, s);
    if (real_fused_generic(p)) return launch_c2r(x, y, tw, sf, p.st, p.numChannels, p.mode, postdiv, s);
    void      *z = nullptr;
-   hipError_t e = device_scratch((size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
+   hipError_t e = call_scratch(d.ws, (size_t) p.st.n * p.numChannels * 8u, s, SCRATCH_REAL, &z);
    if (e == hipSuccess) e = launch_c2r_presplit(x, (float *) z, sf, p.st.n, p.numChannels, p.mode, s);
    if (e == hipSuccess)
       e = run_complex(p.st, (const float *) z, y, tw, p.numChannels,
-                      postdiv ? C2C_INVERSE_POSTDIV : C2C_INVERSE_PRESCALE, s);
+                      postdiv ? C2C_INVERSE_POSTDIV : C2C_INVERSE_PRESCALE, s, d.ws);
    return e;
 }
 
@@ -231,6 +281,33 @@ FFTLIB_STATUS fftlib::describe_call(const Plan &p, const void *pX, const void *p
    default:
       return FFTLIB_ERR_INVALID_VALUE;
    }
+}
+
+FFTLIB_STATUS fftlib::workspace_bytes(const Plan &p, size_t *bytes)
+{
+   size_t need = 0;
+   switch (p.kernel) {
+   case K_C2C_BATCHED:
+   case K_C2C_SINGLE:
+   case K_IC2C_BATCHED:
+   case K_IC2C_SINGLE: need = complex_workspace(p.st, p.numChannels); break;
+   case K_R2C_BATCHED:
+   case K_R2C_SINGLE:
+   case K_C2R_BATCHED:
+   case K_C2R_SINGLE:
+      if (real_two_launch(p)) {
+         const size_t inner = complex_workspace(p.st, p.numChannels);
+         need = inner == SIZE_MAX ? SIZE_MAX : aligned256((size_t) p.st.n * p.numChannels * 8u) + inner;
+      }
+      break;
+   default: break;   // int16: no scratch
+   }
+   if (need == SIZE_MAX) {
+      set_last_error_text("the split-precision C2C variants keep device tables built on first use");
+      return FFTLIB_ERR_NOT_IMPLEMENTED;
+   }
+   *bytes = need;
+   return FFTLIB_SUCCESS;
 }
 
 namespace {

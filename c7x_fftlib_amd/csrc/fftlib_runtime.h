@@ -29,12 +29,33 @@ struct CallBuffers {
    size_t   outBytes;
 };
 
-// Device pointers handed to a launcher.
+// Device memory a call takes its scratch and per-length tables from in
+// place of the library's own buffers (the allocation-free entry point): a
+// region of `bytes` at device address `base`, handed out front to back in
+// 256-byte aligned pieces. A launcher given a workspace allocates nothing.
+struct Workspace {
+   char  *base  = nullptr;
+   size_t bytes = 0;
+   size_t used  = 0;
+   // Next `n` bytes of the region (hipErrorOutOfMemory when they do not fit).
+   hipError_t take(size_t n, void **ptr)
+   {
+      const size_t at = (used + 255u) & ~(size_t) 255u;
+      if (at > bytes || n > bytes - at) return hipErrorOutOfMemory;
+      *ptr = base + at;
+      used = at + n;
+      return hipSuccess;
+   }
+};
+
+// Device pointers handed to a launcher; ws: null = the library's own scratch
+// buffers and tables.
 struct DeviceBuffers {
    const void *in;
    const void *tw;
    const void *aux0;
    void       *out;
+   Workspace  *ws = nullptr;
 };
 
 using LaunchFn = hipError_t (*)(const Plan &plan, const DeviceBuffers &d, hipStream_t stream);
@@ -73,5 +94,14 @@ FFTLIB_STATUS describe_call(const Plan &p, const void *pX, const void *pW, const
 // spectrum of long real transforms (which itself runs a long complex one).
 enum ScratchRole : int { SCRATCH_COMPLEX = 0, SCRATCH_REAL = 1 };
 hipError_t device_scratch(size_t bytes, hipStream_t stream, int role, void **ptr);
+
+// Bytes of workspace one call of plan p takes when it runs with a Workspace
+// (the same route choices as its launcher; 256-byte alignment of every piece
+// included). FFTLIB_ERR_NOT_IMPLEMENTED when the call's route keeps device
+// tables that are built on first use (the split-precision MFMA variants).
+FFTLIB_STATUS workspace_bytes(const Plan &p, size_t *bytes);
+
+// Plan of pBlock as an _init call wrote it (any of the ten fft_c7x kernels).
+FFTLIB_STATUS load_plan(const void *pBlock, Plan *p);
 
 }  // namespace fftlib

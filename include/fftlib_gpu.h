@@ -142,6 +142,31 @@ FFTLIB_STATUS FFTLIB_GPU_pipelineSubmit(FFTLIB_GPU_Pipeline *pipeline, void *pBl
 FFTLIB_STATUS FFTLIB_GPU_pipelineWait(FFTLIB_GPU_Pipeline *pipeline);
 FFTLIB_STATUS FFTLIB_GPU_pipelineDestroy(FFTLIB_GPU_Pipeline *pipeline);
 
+/* Allocation-free variant of FFTLIB_GPU_kernelAsync (no counterpart in the
+ * reference). For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc. Same
+ * arguments, checks and results as FFTLIB_GPU_kernelAsync, but every scratch
+ * buffer a call needs (the intermediate copy of complex transforms longer
+ * than 16384 points, the half-length spectrum of real transforms without a
+ * fused kernel, and the per-length twiddle tables of the long transforms,
+ * which a small kernel writes on every call) comes from one statically
+ * allocated device array of FFTLIB_GPU_NOALLOC_SCRATCH_BYTES per GPU, held
+ * from the library's load until the process ends; no hipMalloc, hipFree or
+ * allocating library routine is called. A call that needs more scratch
+ * returns FFTLIB_ERR_INVALID_DIMENSION before anything is enqueued (a
+ * 65536-point complex transform takes 512 KB of it: up to 127 of them per
+ * call). The split-precision C2C variants keep device tables built on
+ * first use and are refused with FFTLIB_ERR_NOT_IMPLEMENTED.
+ *
+ * Every call on one GPU shares that GPU's array: calls whose work overlaps on
+ * the device (different streams, or several host threads) race on it, so
+ * order them on one stream or synchronise between them. Nothing in the array
+ * needs clearing between calls: each call writes every scratch value before
+ * reading it. */
+#define FFTLIB_GPU_NOALLOC_SCRATCH_BYTES (64u << 20)
+FFTLIB_STATUS FFTLIB_GPU_kernelAsyncNoAlloc(void *pBlock, const void *pX, const void *pW, const void *pAux, void *pY,
+                                            void *hipStream);
+
 /* Waits for all work the library enqueued on its stream. */
 FFTLIB_STATUS FFTLIB_GPU_synchronize(void);
 
