@@ -113,7 +113,7 @@ class Workload:
                        "numPoints": n, "numChannels": b, "variant": self.args.variant,
                        "input": "complex64 uniform[-1,1) Philox seed 960"}
         if self.args.variant != "fp32":
-            self.kernel_name = "c2c960_mfma_kernel<%s>" % self.args.variant
+            self.kernel_name = "c2c_mfma_kernel<%s, 15, 4>" % self.args.variant
         elif self.args.generic:
             self.kernel_name = "c2c_lds_kernel<0, true>"
         else:
@@ -634,7 +634,7 @@ def time_variants(wl, args, lib, F, torch, stream, fp32_ms):
                          "roofline_frac": round(frac, 4), "rel_l2_vs_fp64": chk["rel_l2_vs_fp64"],
                          "bar": VARIANT_BARS[name], "within_bar": chk["rel_l2_vs_fp64"] <= VARIANT_BARS[name],
                          "time_vs_fp32": round(ms / fp32_ms, 4),
-                         "kernel": "c2c960_mfma_kernel<%s>" % name}
+                         "kernel": "c2c_mfma_kernel<%s, 15, 4>" % name}
     finally:
         assert lib.FFTLIB_GPU_setC2CVariant(0) == 0
     return res

@@ -1,26 +1,46 @@
-// Split-precision MFMA variant of the batched complex FFT, N = 960.
+// Split-precision MFMA variants of the batched complex FFT: DFT-as-GEMM on
+// the matrix cores for the lengths N = R1 * 16 * R3 (R1 <= 16, R3 in
+// {1, 2, 4}) listed in FFTLIB_MFMA_LENGTHS below: 21 lengths from 32 to 1024.
 //
-// The 960-point DFT is computed with n = 64a + b (a < 15, b < 64),
-// b = 4 b1 + b2 (b1 < 16, b2 < 4) and k = c + 15 d1 + 240 d2
-// (c < 15, d1 < 16, d2 < 4):
+// With C = 16 R3 columns, n = C a + b (a < R1, b < C), b = R3 b1 + b2
+// (b1 < 16, b2 < R3) and k = c + R1 d1 + 16 R1 d2 (c < R1, d1 < 16, d2 < R3):
 //
-//   Y1[c][b]     = sum_a W15^(ac) x[64a + b]             radix-15 stage (GEMM 1)
-//   Y2[c][b]     = W960^(bc) Y1[c][b]                     twiddle
-//   V[c][b2][d1] = sum_b1 W16^(b1 d1) Y2[c][4 b1 + b2]    radix-16 stage (GEMM 2)
-//   X[c + 15 d1 + 240 d2] = sum_b2 W4^(b2 d2) W64^(b2 d1) V[c][b2][d1]
-//                                                        radix-4 stage (fp32 VALU)
+//   Y1[c][b]     = sum_a W_R1^(ac) x[C a + b]              radix-R1 stage (GEMM 1)
+//   Y2[c][b]     = W_N^(bc) Y1[c][b]                        twiddle
+//   V[c][b2][d1] = sum_b1 W_16^(b1 d1) Y2[c][R3 b1 + b2]    radix-16 stage (GEMM 2)
+//   X[k]         = sum_b2 W_R3^(b2 d2) W_C^(b2 d1) V[c][b2][d1]
+//                                                          radix-R3 stage (fp32 VALU)
 //
-// The radix-15 and radix-16 DFT stages run in real form on
-// v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation; their DFT matrices
-// (32 x 32 real) stay in registers in MFMA operand layout. The data operands
-// are split into sums of 16-bit parts so that the products recover
-// FP32-level accuracy:
+// (for N = 960: the radix-15 and radix-64 = 16 x 4 DFT stages). The outputs
+// go through a per-wavefront LDS staging area so that every store
+// instruction writes 64 consecutive points. Both DFT
+// stages run in real form on v_mfma_f32_16x16x32_{f16,bf16} with FP32
+// accumulation, and both are computed transposed: in GEMM 1 the data is the
+// A operand (rows = the 16 columns b of a tile, K = the real and imaginary
+// parts of x over a) and the DFT matrix the B operand; in GEMM 2 the DFT
+// matrix is the A operand and the data the B operand (K = b1, columns = c).
+// The K order of GEMM 2 is chosen so that the accumulator layout of GEMM 1
+// (work item (q, r) = (lane / 16, lane % 16) holds rows b = 16 mt + 4q + i
+// of column c = r) is exactly the B-operand layout of GEMM 2: for every b2
+// work item q holds four values of b1, and K index 8q + 2e + p is taken to be
+// part p of the e-th of them. The data therefore never leaves the registers
+// between the stages, and the radix-R3 stage finds all b2 of its outputs in
+// one work item. The input is read straight into the GEMM-1 operand layout
+// (K index 8q + 2jj + p is part p of x[C (4q + jj) + b]: four 8-byte loads
+// per tile, whole 128-byte lines per wavefront load). Besides the output
+// staging, LDS holds the DFT-matrix fragments and the W_N table, shared by
+// the workgroup.
+//
+// The data operands are split into sums of 16-bit parts so that the products
+// recover FP32-level accuracy:
 //   FP16x2: x = h + l (fp16), products h*h + h*l + l*h;
 //   BF16x2: same with bf16 parts;
-//   BF16x3: x = h + m + l (bf16), products hh + hm + mh + hl + lh + mm.
+//   BF16x3: x = h + m + l (bf16), products hh + hm + mh + hl + lh + mm
+//           (without mm the N = 960 error against FP64 is 2.6e-6 in a CPU
+//           model of these products).
 // FP16 parts are taken of the transform scaled by a power of two (exact) so
 // that its largest input is in [0.5, 1): fp16 range then holds every operand.
-// The twiddles and the radix-4 stage are fp32.
+// The twiddles and the radix-R3 stage are fp32.
 //
 // This is a different (but equivalent) algorithm from the reference's
 // in-place mixed-radix DIF (src/fft_c7x/FFTLIB_fft1dBatched_i32fc_c32fc_o32fc/
@@ -29,16 +49,17 @@
 // selected with FFTLIB_GPU_setC2CVariant and its error against an FP64 DFT
 // is a property of the split scheme.
 //
-// Work decomposition: 256-thread workgroups, each looping over transforms;
-// each wavefront owns one 9 KB LDS region and runs whole transforms in it
-// (transform w, w + 4*gridDim.x, ...), prefetching the next transform's input
-// into registers while the current one is transformed. The W960 table is
-// copied to LDS once per workgroup.
+// Work decomposition: 256-thread workgroups; each wavefront runs whole
+// transforms (w, w + 4 gridDim.x, ...), loading tile mt of the next transform
+// right after it has split tile mt of the current one.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
-#include <mutex>
 #include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "fft_engine.h"
 #include "fft_kernels.h"
@@ -51,15 +72,8 @@ typedef __bf16   bf168 __attribute__((ext_vector_type(8)));
 typedef float    f32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int      kN        = 960;
-constexpr int      kWaves    = 4;
-constexpr int      kThreadsM = 64 * kWaves;
-constexpr uint32_t kXRow     = 66;          // complex slots per staged input row (64 + pad)
-constexpr uint32_t kVRow     = 36;          // floats per GEMM-2 operand row (re[16] im[16] + pad)
-constexpr uint32_t kRegion   = 64 * kVRow;  // floats per wave region
-constexpr int      kFragA1   = 2 * 64 * 8;  // halves of the two radix-15 A fragments, per part
-constexpr int      kFragB16  = 2 * 64 * 8;  // halves of the two radix-16 B fragments, per part
-static_assert(kRegion >= 15 * kXRow * 2 && kRegion >= 2 * kN, "wave region too small");
+constexpr int kWaves    = 4;
+constexpr int kThreadsM = 64 * kWaves;
 
 enum Split : int { SPLIT_FP16X2 = 0, SPLIT_BF16X2 = 1, SPLIT_BF16X3 = 2 };
 
@@ -120,7 +134,7 @@ __device__ __forceinline__ f32x4 mma(u16x8 a, u16x8 b, f32x4 c)
                                                      0, 0, 0);
 }
 
-// Sum of the split products that the scheme keeps.
+// Sum of the split products that the scheme keeps (smallest terms first).
 template <int S>
 __device__ __forceinline__ f32x4 split_mma(const u16x8 (&a)[SplitTraits<S>::parts],
                                            const u16x8 (&b)[SplitTraits<S>::parts], f32x4 c)
@@ -135,19 +149,17 @@ __device__ __forceinline__ f32x4 split_mma(const u16x8 (&a)[SplitTraits<S>::part
    return mma<S>(a[0], b[0], c);
 }
 
-__device__ __forceinline__ u16x8 neg8(u16x8 v) { return v ^ (uint16_t) 0x8000u; }
-
 // ---- the kernel ---------------------------------------------------------------
 struct MfmaArgs {
    const float2   *x;
    float2         *y;
-   const uint16_t *frag;      // A1 fragments of every part, then B16 fragments of every part
-   const float2   *tw960;     // W960^m, m < 960
+   const uint16_t *frag;      // GEMM-1 B fragments of every part, then GEMM-2 A fragments of every part
+   const float2   *twN;       // W_N^m, m < N
    uint32_t        numChannels;
    int             mode;      // C2CMode
 };
 
-// Wave-wide maximum of |v| over the lanes.
+// Wave-wide maximum of v over the lanes.
 __device__ __forceinline__ float wave_max(float v)
 {
 #pragma unroll
@@ -155,60 +167,96 @@ __device__ __forceinline__ float wave_max(float v)
    return v;
 }
 
+__device__ __forceinline__ float2 cmul(float2 a, float2 w)
+{
+   return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+
+// Halves per fragment table: 2 tiles x parts x 64 lanes x 8.
 template <int S>
-__global__ void __launch_bounds__(kThreadsM) __attribute__((amdgpu_waves_per_eu(3, 3))) c2c960_mfma_kernel(MfmaArgs a)
+constexpr int frag_halves()
+{
+   return 2 * SplitTraits<S>::parts * 64 * 8;
+}
+
+// Outputs are staged in LDS per wavefront and stored 64 consecutive points
+// per instruction: stored straight from registers (runs of R1 points, not
+// line aligned for R1 = 15) N = 960 took 360-405 us instead of 191-198 us.
+template <int S, int R1, int R3>
+constexpr size_t mfma_lds_bytes()
+{
+   return (size_t) 2 * frag_halves<S>() * sizeof(uint16_t) +
+          (size_t) R1 * 16 * R3 * sizeof(float2) * (1 + kWaves);
+}
+
+// Wavefronts per SIMD the register allocation targets: 3 for the shapes
+// with two or four tiles (with 4, the four-tile shapes spill 90-170 bytes per
+// work item and ran 1.5x slower on MI355X), 4 for one tile.
+template <int R3>
+constexpr int mfma_waves_per_eu()
+{
+   return R3 == 1 ? 4 : 3;
+}
+
+template <int S, int R1, int R3>
+__global__ void __launch_bounds__(kThreadsM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R3>())))
+c2c_mfma_kernel(MfmaArgs a)
 {
    constexpr int P = SplitTraits<S>::parts;
-   extern __shared__ float smf[];
-   float2 *tw     = (float2 *) smf;                                       // W960^m, m < 960
-   float  *region = smf + 2 * kN + (threadIdx.x / 64) * kRegion;
-   float2 *rc     = (float2 *) region;
+   constexpr int C = 16 * R3;
+   constexpr int N = R1 * C;
+   constexpr int E = 4 / R3;   // GEMM-2 K elements e of one tile mt: (mt, i) = (e / E, R3 (e % E) + b2)
+   extern __shared__ __align__(16) unsigned char ldsm[];
+   u16x8  *f1 = (u16x8 *) ldsm;   // [t][part][lane]
+   u16x8  *f2 = f1 + 2 * P * 64;
+   float2 *tw = (float2 *) (f2 + 2 * P * 64);
+   for (uint32_t i = threadIdx.x; i < (uint32_t) (4 * P * 64); i += kThreadsM) f1[i] = ((const u16x8 *) a.frag)[i];
+   for (uint32_t i = threadIdx.x; i < (uint32_t) N; i += kThreadsM) tw[i] = a.twN[i];
+   __syncthreads();
 
    const uint32_t lane = threadIdx.x % 64;
    const uint32_t wave = threadIdx.x / 64;
-   const uint32_t q    = lane / 16, r16 = lane % 16;
+   const uint32_t q    = lane / 16, r = lane % 16;
+   const bool     live = r < (uint32_t) R1;   // column c = r exists (the others are padding)
 
-   for (uint32_t i = threadIdx.x; i < (uint32_t) kN; i += kThreadsM) tw[i] = a.tw960[i];
-   // DFT-matrix fragments in registers
-   u16x8 A1[2][P], B16[2][P];
-#pragma unroll
-   for (int p = 0; p < P; ++p)
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-         A1[t][p]  = *(const u16x8 *) (a.frag + p * kFragA1 + (t * 64 + lane) * 8);
-         B16[t][p] = *(const u16x8 *) (a.frag + P * kFragA1 + p * kFragB16 + (t * 64 + lane) * 8);
-      }
-   __syncthreads();
-   // W64^(b2 d1) for d1 = lane % 16, b2 = 1..3 (= W960^(15 b2 d1))
-   float2 w64[3];
-#pragma unroll
-   for (int b2 = 1; b2 < 4; ++b2) w64[b2 - 1] = tw[(15u * (uint32_t) b2 * r16) % (uint32_t) kN];
-
-   const float sxIn  = a.mode == C2C_INVERSE_PRESCALE ? 1.0f / (float) kN : 1.0f;
+   const float sxIn  = a.mode == C2C_INVERSE_PRESCALE ? 1.0f / (float) N : 1.0f;
    const float syIn  = a.mode == C2C_FORWARD ? sxIn : -sxIn;
    const float syOut = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
    const bool  pdiv  = a.mode == C2C_INVERSE_POSTDIV;
 
+   // x[C a + 16 mt + r] of tile mt, a = 4q + jj (zero for a >= R1)
+   auto load_tile = [&](float2(&v)[4], const float2 *xf, int mt) __attribute__((always_inline)) {
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+         const uint32_t aa = 4u * q + (uint32_t) jj;
+         v[jj] = aa < (uint32_t) R1 ? engine::load_stream(xf + C * aa + 16 * mt + r) : make_float2(0.0f, 0.0f);
+      }
+   };
+
    const uint32_t stride = gridDim.x * kWaves;
    uint32_t       f      = blockIdx.x * kWaves + wave;
-   float2         xin[15];
+   float2         v[R3][4];
    if (f < a.numChannels) {
 #pragma unroll
-      for (int j = 0; j < 15; ++j) xin[j] = engine::load_stream(a.x + (size_t) f * kN + lane + 64 * j);
+      for (int mt = 0; mt < R3; ++mt) load_tile(v[mt], a.x + (size_t) f * N, mt);
    }
    for (; f < a.numChannels; f += stride) {
-      // ---- stage the input (x[64a + b] at row a, column b) --------------------
-      float amax = 0.0f;
-#pragma unroll
-      for (int j = 0; j < 15; ++j) {
-         const float2 v = make_float2(sxIn * xin[j].x, syIn * xin[j].y);
-         amax           = fmaxf(amax, fmaxf(fabsf(v.x), fabsf(v.y)));
-         rc[j * kXRow + lane] = v;
-      }
-      // Power-of-two scale bringing max|x| into [0.5, 1) for the fp16 parts
-      // (1 when the transform is zero or not finite, and for bf16).
+      const uint32_t fn   = f + stride;
+      const bool     more = fn < a.numChannels;
+      // ---- input scaling (inverse modes) and the fp16 range scale -----------------
       float scale = 1.0f, unscale = 1.0f;
+      if (a.mode != C2C_FORWARD) {
+#pragma unroll
+         for (int mt = 0; mt < R3; ++mt)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) v[mt][jj] = make_float2(sxIn * v[mt][jj].x, syIn * v[mt][jj].y);
+      }
       if constexpr (S == SPLIT_FP16X2) {
+         float amax = 0.0f;
+#pragma unroll
+         for (int mt = 0; mt < R3; ++mt)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) amax = fmaxf(amax, fmaxf(fabsf(v[mt][jj].x), fabsf(v[mt][jj].y)));
          amax = wave_max(amax);
          if (amax > 0.0f && amax < 1e30f) {
             int e;
@@ -217,89 +265,118 @@ __global__ void __launch_bounds__(kThreadsM) __attribute__((amdgpu_waves_per_eu(
             unscale = ldexpf(1.0f, e);
          }
       }
-      // ---- prefetch the next transform ------------------------------------------
-      const uint32_t fn = f + stride;
-      if (fn < a.numChannels) {
+      // ---- GEMM 1 (transposed): rows b = 16 mt + (0..15), K = parts of x over a ---
+      f32x4 acc1[R3][2];
+      {
+         u16x8 F[2][P];
 #pragma unroll
-         for (int j = 0; j < 15; ++j) xin[j] = engine::load_stream(a.x + (size_t) fn * kN + lane + 64 * j);
-      }
-      // ---- GEMM 1: C1 (32 x 64) = A1 (32 x 32) * B1 (32 x 64) -------------------
-      // B1 row k: k < 15 -> Re x[a = k][b], 16 <= k < 31 -> Im x[a = k - 16][b].
-      f32x4 acc1[2][4];
+         for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-         const uint32_t b = 16 * nt + r16;
-         float          v[8];
+            for (int p = 0; p < P; ++p) F[t][p] = f1[(t * P + p) * 64 + lane];
 #pragma unroll
-         for (int i = 0; i < 8; ++i) {
-            const uint32_t aa = (q & 1u) * 8u + (uint32_t) i;
-            const float2   z  = aa < 15u ? rc[aa * kXRow + b] : make_float2(0.0f, 0.0f);
-            v[i]              = scale * ((q < 2u) ? z.x : z.y);
-         }
-         u16x8 B[P];
-         split8<S>(v, B);
+         for (int mt = 0; mt < R3; ++mt) {
+            float vv[8];
 #pragma unroll
-         for (int mt = 0; mt < 2; ++mt) acc1[mt][nt] = split_mma<S>(A1[mt], B, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-      }
-      // ---- twiddle W960^(bc); Y2 to LDS as rows (c, b2): re[b1], im[b1] ---------
+            for (int jj = 0; jj < 4; ++jj) vv[2 * jj] = scale * v[mt][jj].x, vv[2 * jj + 1] = scale * v[mt][jj].y;
+            u16x8 A[P];
+            split8<S>(vv, A);
+            if (more) load_tile(v[mt], a.x + (size_t) fn * N, mt);   // tile mt of the next transform
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-#pragma unroll
-         for (int i = 0; i < 4; ++i) {
-            const uint32_t c   = 4u * q + (uint32_t) i;   // row 15 is padding (zero)
-            const uint32_t b   = 16u * nt + r16;
-            const float2   w   = tw[(b * c) % (uint32_t) kN];
-            const float    yr  = acc1[0][nt][i], yi = acc1[1][nt][i];
-            float         *row = region + (4u * c + (b & 3u)) * kVRow + (b >> 2);
-            row[0]             = yr * w.x - yi * w.y;
-            row[16]            = yr * w.y + yi * w.x;
+            for (int t = 0; t < 2; ++t) acc1[mt][t] = split_mma<S>(A, F[t], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            __builtin_amdgcn_sched_barrier(0);   // one tile at a time: bounds the live registers
          }
       }
-      // ---- GEMM 2 (radix 16), twiddle W64, radix 4, Z to LDS in output order ---
-      // Row m = 16 mt + lane % 16 of the operand is (c = 4 mt + lane % 16 / 4,
-      // b2 = lane % 4); output rows 16 mt + 4 q + i are (c = 4 mt + q, b2 = i).
-      f32x4 vr[4], vi[4];
+      // ---- twiddle W_N^(b c) and GEMM 2 (transposed) --------------------------------
+      // Element (mt, i) of GEMM 1 is row b = 16 mt + 4q + i of column c = r.
+      // GEMM 2 N-tile u = b2 takes K index 8q + 2e + p = part p of element
+      // (mt, i) = (e / E, R3 (e % E) + u), twiddled as it is consumed.
+      f32x4 acc2[R3][2];
+      {
+         u16x8 F[2][P];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-         const float4 *src = (const float4 *) (region + (16u * mt + r16) * kVRow + 8u * q);
-         const float4  lo = src[0], hi = src[1];
-         float         v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};   // already scaled
-         u16x8         A[P];
-         split8<S>(v, A);
-         vr[mt] = split_mma<S>(A, B16[0], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-         vi[mt] = split_mma<S>(A, B16[1], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-      }
+         for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-         const uint32_t c = 4u * mt + q;
-         if (c >= 15u) continue;
-         // t[b2] = W64^(b2 d1) V[c][b2][d1], d1 = lane % 16
-         float2 t[4];
-         t[0] = make_float2(vr[mt][0], vi[mt][0]);
+            for (int p = 0; p < P; ++p) F[t][p] = f2[(t * P + p) * 64 + lane];
 #pragma unroll
-         for (int b2 = 1; b2 < 4; ++b2) {
-            const float2 w = w64[b2 - 1];
-            t[b2]          = make_float2(vr[mt][b2] * w.x - vi[mt][b2] * w.y, vr[mt][b2] * w.y + vi[mt][b2] * w.x);
-         }
-         const float2 s02 = make_float2(t[0].x + t[2].x, t[0].y + t[2].y);
-         const float2 d02 = make_float2(t[0].x - t[2].x, t[0].y - t[2].y);
-         const float2 s13 = make_float2(t[1].x + t[3].x, t[1].y + t[3].y);
-         const float2 d13 = make_float2(t[1].x - t[3].x, t[1].y - t[3].y);
-         float2       z[4];
-         z[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
-         z[1] = make_float2(d02.x + d13.y, d02.y - d13.x);   // d02 - i d13
-         z[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
-         z[3] = make_float2(d02.x - d13.y, d02.y + d13.x);   // d02 + i d13
+         for (int u = 0; u < R3; ++u) {
+            float vv[8];
 #pragma unroll
-         for (int d2 = 0; d2 < 4; ++d2) {
-            const float zr = unscale * z[d2].x, zi = unscale * z[d2].y;
-            rc[c + 15u * r16 + 240u * (uint32_t) d2] =
-               pdiv ? make_float2(zr / (float) kN, -(zi / (float) kN)) : make_float2(zr, syOut * zi);
+            for (int e = 0; e < 4; ++e) {
+               const int      mt = e / E, i = R3 * (e % E) + u;
+               const uint32_t b  = 16u * mt + 4u * q + (uint32_t) i;
+               const float2   z  = cmul(make_float2(acc1[mt][0][i], acc1[mt][1][i]), tw[live ? b * r : 0u]);
+               vv[2 * e]         = z.x;
+               vv[2 * e + 1]     = z.y;
+            }
+            u16x8 B[P];
+            split8<S>(vv, B);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) acc2[u][t] = split_mma<S>(F[t], B, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            __builtin_amdgcn_sched_barrier(0);   // one tile at a time: bounds the live registers
          }
       }
-      float2 *yo = a.y + (size_t) f * kN;
+      // ---- radix-R3 stage with W_C^(b2 d1) = W_N^(R1 b2 d1); store ----------------
+      float2 *yo = a.y + (size_t) f * N;
+      float2 *zs = tw + N + wave * N;   // this wavefront's output staging
 #pragma unroll
-      for (int j = 0; j < 15; ++j) engine::store_stream(yo + lane + 64 * j, rc[lane + 64 * j]);
+      for (int i = 0; i < 4; ++i) {
+         const uint32_t d1 = 4u * q + (uint32_t) i;
+         float2         t[R3];
+#pragma unroll
+         for (int u = 0; u < R3; ++u) {
+            t[u] = make_float2(acc2[u][0][i], acc2[u][1][i]);
+            if (u > 0) t[u] = cmul(t[u], tw[(uint32_t) (R1 * u) * d1]);
+         }
+         float2 z[R3];
+         if constexpr (R3 == 1) {
+            z[0] = t[0];
+         }
+         else if constexpr (R3 == 2) {
+            z[0] = make_float2(t[0].x + t[1].x, t[0].y + t[1].y);
+            z[1] = make_float2(t[0].x - t[1].x, t[0].y - t[1].y);
+         }
+         else {
+            const float2 s02 = make_float2(t[0].x + t[2].x, t[0].y + t[2].y);
+            const float2 d02 = make_float2(t[0].x - t[2].x, t[0].y - t[2].y);
+            const float2 s13 = make_float2(t[1].x + t[3].x, t[1].y + t[3].y);
+            const float2 d13 = make_float2(t[1].x - t[3].x, t[1].y - t[3].y);
+            z[0]             = make_float2(s02.x + s13.x, s02.y + s13.y);
+            z[1]             = make_float2(d02.x + d13.y, d02.y - d13.x);   // d02 - i d13
+            z[2]             = make_float2(s02.x - s13.x, s02.y - s13.y);
+            z[3]             = make_float2(d02.x - d13.y, d02.y + d13.x);   // d02 + i d13
+         }
+         if (live) {
+#pragma unroll
+            for (int d2 = 0; d2 < R3; ++d2) {
+               const float zr = unscale * z[d2].x, zi = unscale * z[d2].y;
+               const float2 o  = pdiv ? make_float2(zr / (float) N, -(zi / (float) N)) : make_float2(zr, syOut * zi);
+               zs[r + R1 * d1 + 16 * R1 * d2] = o;
+            }
+         }
+      }
+#pragma unroll
+      for (int j = 0; j < (N + 63) / 64; ++j)
+         if (N % 64 == 0 || lane + 64u * j < (uint32_t) N) engine::store_stream(yo + lane + 64 * j, zs[lane + 64 * j]);
+   }
+}
+
+// ---- lengths ------------------------------------------------------------------
+// Every supported length with its factors (R1, R3): the largest R1 <= 16
+// (GEMM 1 uses 2 R1 of its 32 K slots; N = 512 as 8 x 16 x 4 ran at 1.5x the
+// time of 16 x 16 x 2).
+#define FFTLIB_MFMA_LENGTHS(X)                                                                                      \
+   X(32, 2, 1) X(48, 3, 1) X(64, 4, 1) X(80, 5, 1) X(96, 6, 1) X(128, 8, 1) X(144, 9, 1) X(160, 10, 1)             \
+   X(192, 12, 1) X(240, 15, 1) X(256, 16, 1) X(288, 9, 2) X(320, 10, 2) X(384, 12, 2) X(480, 15, 2) X(512, 16, 2) \
+   X(576, 9, 4) X(640, 10, 4) X(768, 12, 4) X(960, 15, 4) X(1024, 16, 4)
+
+bool mfma_factors(uint32_t n, int *r1, int *r3)
+{
+   switch (n) {
+#define FFTLIB_MFMA_CASE(L, A, B)                                                                                   \
+   case L: *r1 = A, *r3 = B; return true;
+      FFTLIB_MFMA_LENGTHS(FFTLIB_MFMA_CASE)
+#undef FFTLIB_MFMA_CASE
+   default: return false;
    }
 }
 
@@ -334,137 +411,154 @@ void host_split(int S, double v, uint16_t *parts)
    }
 }
 
-struct DeviceTables {
-   uint16_t *frag[3]  = {nullptr, nullptr, nullptr};
-   float2   *tw960    = nullptr;
-};
-
-std::mutex   g_tabMutex;
-DeviceTables g_tables[16];
-
-// A1 (32 x 32, real form of the 15-point DFT; rows/columns 15 and 31 are
-// padding) and B16 (32 x 32, real form of the 16-point DFT) in MFMA operand
-// layout for lane l: A[m = 16 t + l % 16][k = 8 (l / 16) + i],
-// B[k = 8 (l / 16) + i][n = 16 t + l % 16]. Layout: A1 of every part, then
-// B16 of every part.
-hipError_t build_tables(int S, uint16_t **out)
+// Real-form DFT coefficient: part t of sum_j W_m^(jk) z[j] contributed by
+// part p of z[j] (jk = the exponent j k).
+double real_form(int t, int p, int jk, int m)
 {
-   const int    P   = S == SPLIT_BF16X3 ? 3 : 2;
-   const size_t cnt = (size_t) P * (kFragA1 + kFragB16);
-   uint16_t    *h   = new uint16_t[cnt];
    const double pi  = 3.14159265358979323846;
-   uint16_t     parts[3];
-   auto A1 = [&](int m, int k) -> double {
-      const int c = m & 15, a = k & 15;
-      if (c >= 15 || a >= 15) return 0.0;
-      const double ang = -2.0 * pi * (double) ((a * c) % 15) / 15.0;
-      const double fr = cos(ang), fi = sin(ang);
-      if (m < 16) return k < 16 ? fr : -fi;   // Re out = fr Re in - fi Im in
-      return k < 16 ? fi : fr;                // Im out = fi Re in + fr Im in
-   };
-   auto B16 = [&](int k, int n) -> double {
-      const int    b1 = k & 15, d1 = n & 15;
-      const double ang = -2.0 * pi * (double) ((b1 * d1) % 16) / 16.0;
-      const double fr = cos(ang), fi = sin(ang);
-      if (n < 16) return k < 16 ? fr : -fi;
-      return k < 16 ? fi : fr;
-   };
-   for (int t = 0; t < 2; ++t)
-      for (int l = 0; l < 64; ++l)
-         for (int i = 0; i < 8; ++i) {
-            host_split(S, A1(16 * t + l % 16, 8 * (l / 16) + i), parts);
-            for (int p = 0; p < P; ++p) h[(size_t) p * kFragA1 + (t * 64 + l) * 8 + i] = parts[p];
-            host_split(S, B16(8 * (l / 16) + i, 16 * t + l % 16), parts);
-            for (int p = 0; p < P; ++p)
-               h[(size_t) P * kFragA1 + (size_t) p * kFragB16 + (t * 64 + l) * 8 + i] = parts[p];
-         }
-   uint16_t  *d = nullptr;
-   hipError_t e = hipMalloc((void **) &d, cnt * sizeof(uint16_t));
-   if (e == hipSuccess) e = hipMemcpy(d, h, cnt * sizeof(uint16_t), hipMemcpyHostToDevice);
-   delete[] h;
-   if (e != hipSuccess) {
-      if (d) (void) hipFree(d);
-      return e;
-   }
-   *out = d;
-   return hipSuccess;
+   const double ang = -2.0 * pi * (double) (jk % m) / (double) m;
+   const double wr = cos(ang), wi = sin(ang);
+   if (t == 0) return p == 0 ? wr : -wi;
+   return p == 0 ? wi : wr;
 }
 
-hipError_t get_tables(int S, const uint16_t **frag, const float2 **tw)
+// Fragment tables of one (split, R1, R3) in MFMA operand layout for lane
+// l = 16 q + r, tile t and half j (e = j >> 1, p = j & 1):
+//   GEMM-1 B operand: B[k = 8q + j][n = 16t + r] = coefficient of part p of
+//     x[a = 4q + e] in part t of Y1[c = r] (0 when a or c is at least R1);
+//   GEMM-2 A operand: A[m = 16t + r][k = 8q + j] = coefficient of part p of
+//     Y2[b1] in part t of V[d1 = r], b1 = 16 mt / R3 + 4q / R3 + e % E with
+//     mt = e / E, E = 4 / R3 (the b1 of element e of work item q).
+// Layout: [table][t][part][lane][8 halves].
+std::vector<uint16_t> build_frags(int S, int R1, int R3)
 {
-   int dev = 0;
-   hipError_t e = hipGetDevice(&dev);
+   const int             P = S == SPLIT_BF16X3 ? 3 : 2;
+   const int             E = 4 / R3;
+   std::vector<uint16_t> h((size_t) 2 * 2 * P * 64 * 8);
+   uint16_t              parts[3];
+   for (int tab = 0; tab < 2; ++tab)
+      for (int t = 0; t < 2; ++t)
+         for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < 8; ++j) {
+               const int q = l / 16, r = l % 16, p = j & 1, e = j >> 1;
+               double    v = 0.0;
+               if (tab == 0) {
+                  const int aa = 4 * q + e;
+                  if (aa < R1 && r < R1) v = real_form(t, p, aa * r, R1);
+               }
+               else {
+                  const int b1 = 16 * (e / E) / R3 + 4 * q / R3 + e % E;
+                  v            = real_form(t, p, b1 * r, 16);
+               }
+               host_split(S, v, parts);
+               for (int pp = 0; pp < P; ++pp) h[((((size_t) tab * 2 + t) * P + pp) * 64 + l) * 8 + j] = parts[pp];
+            }
+   return h;
+}
+
+struct TableKey {
+   int      dev, split;
+   uint32_t n;
+   bool     operator<(const TableKey &o) const
+   {
+      return dev != o.dev ? dev < o.dev : split != o.split ? split < o.split : n < o.n;
+   }
+};
+struct DeviceTables {
+   uint16_t *frag = nullptr;
+   float2   *twN  = nullptr;
+};
+std::mutex                       g_tabMutex;
+std::map<TableKey, DeviceTables> g_tables;   // device tables live until the process ends
+
+hipError_t get_tables(int S, uint32_t n, int R1, int R3, const uint16_t **frag, const float2 **twN)
+{
+   int        dev = 0;
+   hipError_t e   = hipGetDevice(&dev);
    if (e != hipSuccess) return e;
-   if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
    std::lock_guard<std::mutex> lock(g_tabMutex);
-   DeviceTables &t = g_tables[dev];
-   if (t.tw960 == nullptr) {
-      float2      *h  = new float2[kN];
-      const double pi = 3.14159265358979323846;
-      for (int m = 0; m < kN; ++m) {
-         const double ang = -2.0 * pi * (double) m / (double) kN;
-         h[m]             = make_float2((float) cos(ang), (float) sin(ang));
+   auto it = g_tables.find(TableKey{dev, S, n});
+   if (it == g_tables.end()) {
+      const std::vector<uint16_t> h = build_frags(S, R1, R3);
+      std::vector<float2>         w(n);
+      const double                pi = 3.14159265358979323846;
+      for (uint32_t m = 0; m < n; ++m) {
+         const double ang = -2.0 * pi * (double) m / (double) n;
+         w[m]             = make_float2((float) cos(ang), (float) sin(ang));
       }
-      float2 *d = nullptr;
-      e         = hipMalloc((void **) &d, kN * sizeof(float2));
-      if (e == hipSuccess) e = hipMemcpy(d, h, kN * sizeof(float2), hipMemcpyHostToDevice);
-      delete[] h;
-      if (e != hipSuccess) {
-         if (d) (void) hipFree(d);
+      DeviceTables t;
+      if ((e = hipMalloc((void **) &t.frag, h.size() * sizeof(uint16_t))) != hipSuccess) return e;
+      if ((e = hipMalloc((void **) &t.twN, n * sizeof(float2))) != hipSuccess) {
+         (void) hipFree(t.frag);
          return e;
       }
-      t.tw960 = d;
+      if ((e = hipMemcpy(t.frag, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) != hipSuccess ||
+          (e = hipMemcpy(t.twN, w.data(), n * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
+         (void) hipFree(t.frag);
+         (void) hipFree(t.twN);
+         return e;
+      }
+      it = g_tables.emplace(TableKey{dev, S, n}, t).first;
    }
-   if (t.frag[S] == nullptr) {
-      e = build_tables(S, &t.frag[S]);
-      if (e != hipSuccess) return e;
-   }
-   *frag = t.frag[S];
-   *tw   = t.tw960;
+   *frag = it->second.frag;
+   *twN  = it->second.twN;
    return hipSuccess;
 }
 
-template <int S>
-hipError_t launch_s(const MfmaArgs &a, hipStream_t stream)
+// Grid: `perWave` = transforms per wavefront the grid aims for (0 = the
+// resident grid, every wavefront looping over numChannels / (resident
+// wavefronts) transforms).
+template <int S, int R1, int R3>
+hipError_t launch_form(const MfmaArgs &a, uint32_t perWave, hipStream_t stream)
 {
-   constexpr int P   = SplitTraits<S>::parts;
-   const size_t  lds = kN * sizeof(float2) + (size_t) kWaves * kRegion * sizeof(float);
-   static int resident = 0;
-   if (lds > 65536) {
-      const hipError_t e = hipFuncSetAttribute((const void *) c2c960_mfma_kernel<S>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-      if (e != hipSuccess) return e;
-   }
+   auto             kern     = c2c_mfma_kernel<S, R1, R3>;
+   constexpr size_t lds      = mfma_lds_bytes<S, R1, R3>();
+   static int       resident = 0;
    if (resident == 0) {
       int dev = 0, cus = 0, per = 0;
       if (hipGetDevice(&dev) != hipSuccess ||
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
          cus = 256;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, c2c960_mfma_kernel<S>, kThreadsM, lds) != hipSuccess ||
-          per <= 0)
-         per = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kThreadsM, lds) != hipSuccess || per <= 0) per = 1;
       resident = cus * per;
    }
-   // Grid: fp16x2 and bf16x2 run about 7% faster with about 4 transforms per
-   // wavefront (65536 transforms: 4096 workgroups, fp16x2 187.6 -> 173.6 us,
-   // bf16x2 186.2 -> 173.0 us on MI355X) than with the resident grid (768
-   // workgroups, 21 per wavefront); bf16x3, with twice the matrix products,
-   // was fastest with the resident grid.
    const uint32_t need = (a.numChannels + kWaves - 1) / kWaves;
    uint32_t       grid = need < (uint32_t) resident ? need : (uint32_t) resident;
-   if (S != SPLIT_BF16X3 && (need + 3u) / 4u > grid) grid = (need + 3u) / 4u;
-   hipLaunchKernelGGL(c2c960_mfma_kernel<S>, dim3(grid), dim3(kThreadsM), lds, stream, a);
+   if (perWave != 0 && (need + perWave - 1u) / perWave > grid) grid = (need + perWave - 1u) / perWave;
+   hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreadsM), lds, stream, a);
    return hipGetLastError();
 }
 
+template <int S>
+hipError_t launch_split(uint32_t n, const MfmaArgs &a, uint32_t perWave, hipStream_t stream)
+{
+   switch (n) {
+#define FFTLIB_MFMA_CASE(L, A, B)                                                                                   \
+   case L: return launch_form<S, A, B>(a, perWave, stream);
+      FFTLIB_MFMA_LENGTHS(FFTLIB_MFMA_CASE)
+#undef FFTLIB_MFMA_CASE
+   default: return hipErrorInvalidValue;
+   }
+}
+
+// Transforms per wavefront the launches aim for (0 = the resident grid):
+// about 4 per wavefront ran 2-7% faster than the resident grid at 240-960
+// points (N = 960 BF16x3: 198.4 -> 191.0 us, FP16x2: 183.3 -> 172.9 us).
+constexpr uint32_t kMfmaPerWave = 4;
+
 }  // namespace
 
-bool mfma_c2c_supported(uint32_t n) { return n == (uint32_t) kN; }
-
-hipError_t launch_c2c_mfma(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, int variant,
-                           hipStream_t stream)
+bool mfma_c2c_supported(uint32_t n)
 {
-   if (n != (uint32_t) kN) return hipErrorInvalidValue;
+   int r1, r3;
+   return mfma_factors(n, &r1, &r3);
+}
+
+hipError_t launch_c2c_mfma_grid(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, int variant,
+                                uint32_t perWave, hipStream_t stream)
+{
+   int R1 = 0, R3 = 0;
+   if (!mfma_factors(n, &R1, &R3)) return hipErrorInvalidValue;
    if (numChannels == 0) return hipSuccess;
    int S;
    switch (variant) {
@@ -474,12 +568,18 @@ hipError_t launch_c2c_mfma(const float *x, float *y, uint32_t n, uint32_t numCha
    default: return hipErrorInvalidValue;
    }
    MfmaArgs   a{};
-   hipError_t e = get_tables(S, &a.frag, &a.tw960);
+   hipError_t e = get_tables(S, n, R1, R3, &a.frag, &a.twN);
    if (e != hipSuccess) return e;
    a.x = (const float2 *) x, a.y = (float2 *) y, a.numChannels = numChannels, a.mode = mode;
-   if (S == SPLIT_FP16X2) return launch_s<SPLIT_FP16X2>(a, stream);
-   if (S == SPLIT_BF16X2) return launch_s<SPLIT_BF16X2>(a, stream);
-   return launch_s<SPLIT_BF16X3>(a, stream);
+   if (S == SPLIT_FP16X2) return launch_split<SPLIT_FP16X2>(n, a, perWave, stream);
+   if (S == SPLIT_BF16X2) return launch_split<SPLIT_BF16X2>(n, a, perWave, stream);
+   return launch_split<SPLIT_BF16X3>(n, a, perWave, stream);
+}
+
+hipError_t launch_c2c_mfma(const float *x, float *y, uint32_t n, uint32_t numChannels, int mode, int variant,
+                           hipStream_t stream)
+{
+   return launch_c2c_mfma_grid(x, y, n, numChannels, mode, variant, kMfmaPerWave, stream);
 }
 
 }  // namespace fftlib

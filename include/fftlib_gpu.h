@@ -40,14 +40,16 @@ FFTLIB_STATUS FFTLIB_GPU_setSynchronous(int32_t synchronous);
 FFTLIB_STATUS FFTLIB_GPU_setSpecializedKernels(int32_t enable);
 
 /* Arithmetic of the fp32 complex FFT kernels (batched and single, forward and
- * inverse) for lengths that have a DFT-as-GEMM matrix-core kernel (N = 960);
- * other lengths always use FFTLIB_C2C_VARIANT_FP32.
+ * inverse) for the lengths that have a DFT-as-GEMM matrix-core kernel:
+ * N = R1 * 16 * R3 with R1 <= 16 and R3 in {1, 2, 4}, namely 32, 48, 64, 80,
+ * 96, 128, 144, 160, 192, 240, 256, 288, 320, 384, 480, 512, 576, 640, 768,
+ * 960 and 1024; other lengths always use FFTLIB_C2C_VARIANT_FP32.
  *   FFTLIB_C2C_VARIANT_FP32   (0, default) mixed-radix fp32, the reference's
  *                             algorithm and twiddle table;
- *   FFTLIB_C2C_VARIANT_BF16X3 (1) radix-15 and radix-16 DFT stages as MFMA
+ *   FFTLIB_C2C_VARIANT_BF16X3 (1) radix-R1 and radix-16 DFT stages as MFMA
  *                             GEMMs on bf16 hi+mid+lo operand parts (6
  *                             products), FP32 accumulation, then an fp32
- *                             radix-4 stage;
+ *                             radix-R3 stage;
  *   FFTLIB_C2C_VARIANT_FP16X2 (2) same with fp16 hi+lo parts (3 products);
  *   FFTLIB_C2C_VARIANT_BF16X2 (3) same with bf16 hi+lo parts (3 products).
  * The MFMA variants generate their own DFT matrices and do not read pW.

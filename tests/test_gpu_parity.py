@@ -245,10 +245,13 @@ def test_real_single(n):
     assert G.rel_l2(z, refz) <= REL_TOL
 
 
-# Split-precision MFMA variants of the batched / single C2C (N = 960): error
-# against an FP64 DFT. FP16x2 and BF16x3 meet the 2e-6 bar; BF16x2 (3 bf16
-# products) is reported with its own, looser bound.
+# Split-precision MFMA variants of the batched / single C2C: error against an
+# FP64 DFT. FP16x2 and BF16x3 meet the 2e-6 bar; BF16x2 (3 bf16 products) is
+# held to its own, looser bound. Every length with an MFMA kernel (the three
+# shapes R3 = 1, 2, 4 of N = R1 x 16 x R3, R1 = 2 .. 16), batch counts that
+# leave a partly filled last workgroup.
 MFMA_VARIANTS = {1: 2e-6, 2: 2e-6, 3: 1e-5}
+MFMA_LENGTHS = [32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256, 288, 320, 384, 480, 512, 576, 640, 768, 960, 1024]
 
 
 @pytest.fixture
@@ -260,12 +263,12 @@ def c2c_variant():
 
 @pytest.mark.parametrize("kind", ["fft", "ifft", "ifft_single"])
 @pytest.mark.parametrize("variant", sorted(MFMA_VARIANTS))
-def test_c2c_mfma_variants(variant, kind, c2c_variant):
+@pytest.mark.parametrize("n", MFMA_LENGTHS)
+def test_c2c_mfma_variants(n, variant, kind, c2c_variant):
     assert c2c_variant(variant) == 0
-    n = 960
     single = kind == "ifft_single"
     b = 1 if single else 77
-    x = cplx(rng(variant * 31 + len(kind)), b, n, 1.0)
+    x = cplx(rng(variant * 31 + len(kind) + n), b, n, 1.0)
     s, y = G.c2c(x, n, b, inverse=kind != "fft", single=single)
     assert s == 0, G.F.last_error()
     x64 = x.astype(np.complex128)
@@ -273,11 +276,12 @@ def test_c2c_mfma_variants(variant, kind, c2c_variant):
     assert G.rel_l2(y, ref) <= MFMA_VARIANTS[variant]
 
 
+@pytest.mark.parametrize("n", [240, 480, 960])
 @pytest.mark.parametrize("variant", [1, 2])
-def test_c2c_mfma_scaled_inputs(variant, c2c_variant):
+def test_c2c_mfma_scaled_inputs(variant, n, c2c_variant):
     """Inputs far from unit magnitude (fp16 parts need the power-of-two scaling)."""
     assert c2c_variant(variant) == 0
-    n, b = 960, 6
+    b = 6
     x = cplx(rng(99 + variant), b, n, 1.0) * np.array([1e-6, 1e-3, 1.0, 3e2, 6e4, 2e7], np.float32)[:, None]
     s, y = G.c2c(x.astype(np.complex64), n, b)
     assert s == 0, G.F.last_error()
@@ -288,7 +292,7 @@ def test_c2c_mfma_scaled_inputs(variant, c2c_variant):
 
 def test_c2c_variant_other_lengths_use_fp32(c2c_variant):
     """Lengths without an MFMA kernel keep the fp32 path (bit-identical results)."""
-    n, b = 1024, 3
+    n, b = 1000, 3
     x = cplx(rng(5), b, n, 1.0)
     s0, y0 = G.c2c(x, n, b)
     assert c2c_variant(2) == 0
