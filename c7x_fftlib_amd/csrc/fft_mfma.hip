@@ -35,9 +35,10 @@
 // recover FP32-level accuracy:
 //   FP16x2: x = h + l (fp16), products h*h + h*l + l*h;
 //   BF16x2: same with bf16 parts;
-//   BF16x3: x = h + m + l (bf16), products hh + hm + mh + hl + lh + mm
-//           (without mm the N = 960 error against FP64 is 2.6e-6 in a CPU
-//           model of these products).
+//   BF16x3: x = h + m + l (bf16; the data split by truncation, which is
+//           exact, the matrices rounded), products hh + hm + mh + hl + lh
+//           + mm (without mm the N = 960 error against FP64 is 2.6e-6 in a
+//           CPU model of these products).
 // FP16 parts are taken of the transform scaled by a power of two (exact) so
 // that its largest input is in [0.5, 1): fp16 range then holds every operand.
 // The twiddles and the radix-R3 stage are fp32.
@@ -96,10 +97,44 @@ __device__ __forceinline__ uint16_t bf16_bits(float v)
 }
 __device__ __forceinline__ float bf16_value(uint16_t b) { return (float) __builtin_bit_cast(__bf16, b); }
 
+// Upper halves of two floats as one register (bf16 pair: a low, b high).
+__device__ __forceinline__ uint32_t hi_halves(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x07060302u); }
+
 // Splits 8 floats into the parts of scheme S (part 0 = high).
+// BF16x3 splits by truncation, which is exact: h = x with the low 16 bits
+// cleared, r = x - h (exact: the low 16 mantissa bits of x), m = r truncated
+// the same way, l = r - m (exact, at most 8 significant bits, so a bf16
+// value); each part is the upper half of its float. Two bit masks, two
+// subtractions and 1.5 permutes per value, against 7.5 instructions for the
+// rounding conversions the compiler emits for (__bf16) casts.
 template <int S>
 __device__ __forceinline__ void split8(const float (&v)[8], u16x8 (&p)[SplitTraits<S>::parts])
 {
+   if constexpr (S == SPLIT_BF16X3) {
+      uint32_t ph[4], pm[4], pl[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+         uint32_t h[2], m[2], l[2];
+#pragma unroll
+         for (int k = 0; k < 2; ++k) {
+            const float x  = v[2 * w + k];
+            h[k]           = __float_as_uint(x) & 0xFFFF0000u;
+            const float rr = x - __uint_as_float(h[k]);
+            m[k]           = __float_as_uint(rr) & 0xFFFF0000u;
+            l[k]           = __float_as_uint(rr - __uint_as_float(m[k]));
+            h[k]           = __float_as_uint(x);
+            m[k]           = __float_as_uint(rr);
+         }
+         ph[w] = hi_halves(h[0], h[1]);
+         pm[w] = hi_halves(m[0], m[1]);
+         pl[w] = hi_halves(l[0], l[1]);
+      }
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      p[0] = __builtin_bit_cast(u16x8, (u32x4){ph[0], ph[1], ph[2], ph[3]});
+      p[1] = __builtin_bit_cast(u16x8, (u32x4){pm[0], pm[1], pm[2], pm[3]});
+      p[2] = __builtin_bit_cast(u16x8, (u32x4){pl[0], pl[1], pl[2], pl[3]});
+      return;
+   }
 #pragma unroll
    for (int i = 0; i < 8; ++i) {
       if constexpr (S == SPLIT_FP16X2) {
