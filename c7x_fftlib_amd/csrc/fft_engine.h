@@ -99,11 +99,20 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f)
 // ..._cn.c:368-381). TW = false is the last stage of a transform (wq == 1,
 // p == 0), whose twiddles are the table's W^0 = (1, -0) entries: the
 // products are skipped (they change at most the sign of a zero).
+// Entry i of a table: the base plus a 32-bit byte offset, so that loads from
+// global memory take the table's (uniform) address from scalar registers
+// and one 32-bit offset register each; with the offset scaled in 64 bits
+// every twiddle load held its own 64-bit address (N = 16384: spills).
+__device__ __forceinline__ float2 table_at(const float2 *w, uint32_t i)
+{
+   return *(const float2 *) ((const char *) w + i * (uint32_t) sizeof(float2));
+}
+
 template <int R, bool TW = true>
 __device__ __forceinline__ void butterfly(float2 *v, const int o, const int S, const float2 *w, uint32_t wq,
                                           uint32_t p)
 {
-   auto tw = [&](float2 a, uint32_t i) { return TW ? cmul(a, w[i]) : a; };
+   auto tw = [&](float2 a, uint32_t i) { return TW ? cmul(a, table_at(w, i)) : a; };
    if constexpr (R == 2) {
       const float2 a = v[o], b = v[o + S];
       v[o]           = add2(a, b);

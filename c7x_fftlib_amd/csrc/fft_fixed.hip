@@ -94,8 +94,8 @@ bool main_c2c_has(uint32_t n)
 uint32_t fixed_fpb(uint32_t n)
 {
    uint32_t T = 1;
-   while (T < 256u && T * 16u < n) T *= 2u;
-   return (uint32_t) kFixedThreads / T;
+   while (T < (uint32_t) kMaxFixedThreads && T * 16u < n) T *= 2u;
+   return T >= (uint32_t) kFixedThreads ? 1u : (uint32_t) kFixedThreads / T;
 }
 
 // One task of a class kernel covering `count` channels of length n.

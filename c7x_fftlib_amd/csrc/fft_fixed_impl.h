@@ -17,7 +17,8 @@ namespace fixedk {
 
 using namespace engine;
 
-inline constexpr int kFixedThreads = 256;
+inline constexpr int kFixedThreads = 256;    // work items per workgroup up to N = 4096
+inline constexpr int kMaxFixedThreads = 1024; // longer lengths: one transform of N / 16 work items
 inline constexpr int kMaxFixedSt   = 24;
 
 enum FixedKind : int { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
@@ -40,6 +41,7 @@ struct FixedPlan {
    int  twCount;           // complex twiddles the stages read
    int  T;                 // work items per transform
    int  FPB;               // transforms per workgroup
+   int  threads;           // work items per workgroup: T * FPB
 };
 
 constexpr inline FixedPlan make_fixed_plan(int n)
@@ -76,10 +78,15 @@ constexpr inline FixedPlan make_fixed_plan(int n)
       ++p.nPasses;
       s += pair ? 2 : 1;
    }
+   // at most 16 points per work item: 256-item workgroups of FPB transforms
+   // up to N = 4096, then one transform per workgroup of N / 16 (512 or
+   // 1024) work items, so that the LDS passes of the longest lengths (one
+   // workgroup per CU) still run 8 or 16 waves per CU
    int T = 1;
-   while (T < 256 && T * 16 < n) T *= 2;
-   p.T   = T;
-   p.FPB = kFixedThreads / T;
+   while (T < kMaxFixedThreads && T * 16 < n) T *= 2;
+   p.T       = T;
+   p.threads = T > kFixedThreads ? T : kFixedThreads;
+   p.FPB     = p.threads / T;
    return p;
 }
 
@@ -499,6 +506,7 @@ constexpr bool r2c_group_store()
 template <int N>
 __device__ __forceinline__ void r2c_store_group(const float2 *smem, uint32_t first, uint32_t nf, const FixedArgs &a)
 {
+   static_assert(Fixed<N>::P.threads == kFixedThreads, "groups of several transforms: 256 work items");
    constexpr int      FPB    = Fixed<N>::P.FPB;
    constexpr uint32_t STRIDE = fixed_stride<N>();
    constexpr int      iters  = (N + kFixedThreads - 1) / kFixedThreads;
@@ -558,6 +566,7 @@ constexpr bool c2c_linear_store()
 template <int N>
 __device__ __forceinline__ void c2c_store_linear(const float2 *smem, uint32_t first, uint32_t nf, const FixedArgs &a)
 {
+   static_assert(Fixed<N>::P.threads == kFixedThreads, "groups of several transforms: 256 work items");
    constexpr int      FPB    = Fixed<N>::P.FPB;
    constexpr uint32_t STRIDE = fixed_stride<N>();
    constexpr int      IT     = FPB * N / kFixedThreads;
@@ -603,6 +612,7 @@ constexpr bool r2c_linear_store()
 template <int N>
 __device__ __forceinline__ void r2c_store_linear(const float2 *smem, uint32_t first, uint32_t nf, const FixedArgs &a)
 {
+   static_assert(Fixed<N>::P.threads == kFixedThreads, "groups of several transforms: 256 work items");
    constexpr int      FPB    = Fixed<N>::P.FPB;
    constexpr uint32_t STRIDE = fixed_stride<N>();
    constexpr uint32_t ROW    = (uint32_t) N + 1u;
@@ -674,7 +684,8 @@ template <int N, int KIND, uint32_t STRIDE>
 __device__ __forceinline__ void stage_group(const FixedArgs &a, uint32_t first, uint32_t nf, InScale sc, float2 *smem)
 {
    constexpr int PTS = Fixed<N>::P.FPB * N;
-   constexpr int IT  = (PTS + kFixedThreads - 1) / kFixedThreads;
+   constexpr int TH  = Fixed<N>::P.threads;
+   constexpr int IT  = (PTS + TH - 1) / TH;
    if constexpr (KIND != KIND_C2R && IT <= 16) {
       // The two address forms are separate uniform paths: with the offset
       // table looked up per point inside one loop, every data load waited
@@ -682,16 +693,31 @@ __device__ __forceinline__ void stage_group(const FixedArgs &a, uint32_t first, 
       // data loads), 16 HBM round trips per group (N = 256: 123 us per 2^25
       // points).
       float2 v[IT];
-      if (a.inOff) {
+      if (a.inOff && Fixed<N>::P.FPB <= 64) {
+         // lane j of each wave loads the offset of the group's transform j;
+         // the others come by lane shuffle (LDS crossbar, no memory wait)
+         const uint32_t lane = threadIdx.x % 64u;
+         const uint64_t mine = a.inOff[first + (lane < nf ? lane : 0u)];
+         const uint32_t lo = (uint32_t) mine, hi = (uint32_t) (mine >> 32);
+#pragma unroll
+         for (int c = 0; c < IT; ++c) {
+            const uint32_t o  = threadIdx.x + (uint32_t) (c * TH);
+            const uint32_t f  = o / (uint32_t) N;
+            const int      fs = (int) (f < nf ? f : 0u);
+            const uint64_t b  = ((uint64_t) (uint32_t) __shfl((int) hi, fs) << 32) | (uint32_t) __shfl((int) lo, fs);
+            v[c]              = fixed_input<N, KIND>(a.x + b, o - f * (uint32_t) N, a);
+         }
+      }
+      else if (a.inOff) {
          uint64_t off[IT];
 #pragma unroll
          for (int c = 0; c < IT; ++c) {
-            const uint32_t f = (threadIdx.x + (uint32_t) (c * kFixedThreads)) / (uint32_t) N;
+            const uint32_t f = (threadIdx.x + (uint32_t) (c * TH)) / (uint32_t) N;
             off[c]           = a.inOff[first + (f < nf ? f : 0u)];
          }
 #pragma unroll
          for (int c = 0; c < IT; ++c) {
-            const uint32_t o = threadIdx.x + (uint32_t) (c * kFixedThreads);
+            const uint32_t o = threadIdx.x + (uint32_t) (c * TH);
             const uint32_t f = o / (uint32_t) N;
             v[c]             = fixed_input<N, KIND>(a.x + off[c], o - f * (uint32_t) N, a);
          }
@@ -699,7 +725,7 @@ __device__ __forceinline__ void stage_group(const FixedArgs &a, uint32_t first, 
       else {
 #pragma unroll
          for (int c = 0; c < IT; ++c) {
-            const uint32_t o = threadIdx.x + (uint32_t) (c * kFixedThreads);
+            const uint32_t o = threadIdx.x + (uint32_t) (c * TH);
             const uint32_t f = o / (uint32_t) N;
             const size_t   b = (size_t) (first + (f < nf ? f : 0u)) * a.inStride;
             v[c]             = fixed_input<N, KIND>(a.x + b, o - f * (uint32_t) N, a);
@@ -707,14 +733,14 @@ __device__ __forceinline__ void stage_group(const FixedArgs &a, uint32_t first, 
       }
 #pragma unroll
       for (int c = 0; c < IT; ++c) {
-         const uint32_t o = threadIdx.x + (uint32_t) (c * kFixedThreads);
+         const uint32_t o = threadIdx.x + (uint32_t) (c * TH);
          const uint32_t f = o / (uint32_t) N;
          const uint32_t k = o - f * (uint32_t) N;
          if (f < nf) smem[f * STRIDE + fslot(k)] = make_float2(sc.sx * v[c].x, sc.sy * v[c].y);
       }
    }
    else {
-      for (uint32_t i = threadIdx.x; i < nf * N; i += kFixedThreads) {
+      for (uint32_t i = threadIdx.x; i < nf * N; i += TH) {
          const uint32_t f = i / (uint32_t) N;
          const uint32_t k = i - f * (uint32_t) N;
          const float2   z = fixed_input<N, KIND>(a.x + in_base(a, first + f), k, a);
@@ -748,6 +774,7 @@ template <int N>
 __device__ __forceinline__ void c2r_presplit_group(const FixedArgs &a, uint32_t first, uint32_t nf, InScale sc,
                                                    float2 *smem)
 {
+   static_assert(!c2r_group_presplit<N>() || Fixed<N>::P.threads == kFixedThreads, "groups of several transforms: 256 work items");
    constexpr int      FPB    = Fixed<N>::P.FPB;
    constexpr uint32_t STRIDE = fixed_stride<N>();
    constexpr uint32_t H      = (uint32_t) N / 2u;
@@ -804,14 +831,22 @@ constexpr bool fixed_direct()
    return Pass0<N>::q2 >= 32;
 }
 
-// Registers held across the LDS passes by the next group's first-pass inputs
-// are bounded: lengths whose first pass needs more than 16 points per work
-// item load them right before use instead. The C2R pre-split combines four
-// loads per point as they arrive, so C2R does not prefetch.
-template <int N, int KIND>
-constexpr bool fixed_prefetch()
+// Waves per SIMD of a kernel whose workgroups of `threads` work items use
+// `lds` bytes of LDS each (160 KB per CU, four SIMDs), at most 8.
+constexpr int lds_waves_per_simd(size_t lds, int threads)
 {
-   return KIND != KIND_C2R && fixed_direct<N>() && Pass0<N>::iters * Pass0<N>::R <= 16;
+   const size_t wgs = lds ? (size_t) 163840 / lds : 8u;
+   const size_t w   = (wgs < 1 ? 1 : wgs) * (size_t) (threads / 256);
+   return w < 1 ? 1 : w > 8 ? 8 : (int) w;
+}
+
+// Points per work item that a group's first pass may hold in registers (all
+// its inputs loaded at once: one HBM round trip per group): 16 at four or
+// more waves per SIMD, more where fewer waves leave more registers per work
+// item.
+constexpr int pass0_register_points(int wavesPerSimd)
+{
+   return wavesPerSimd >= 4 ? 16 : wavesPerSimd >= 2 ? 32 : 64;
 }
 
 // The twiddle table is staged in LDS (once per workgroup) when it fits next
@@ -831,8 +866,35 @@ constexpr size_t fixed_lds_bytes()
    return ((size_t) P.FPB * fixed_stride<N>() + (fixed_tw_lds<N>() ? (size_t) P.twCount : 0u)) * sizeof(float2);
 }
 
+// Registers held across the LDS passes by the next group's first-pass inputs
+// are bounded: lengths whose first pass needs more points per work item than
+// pass0_register_points allows load them right before use instead. The C2R
+// pre-split combines four loads per point as they arrive, so C2R does not
+// prefetch.
 template <int N, int KIND>
-__global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
+constexpr bool fixed_prefetch()
+{
+   return KIND != KIND_C2R && fixed_direct<N>() &&
+          Pass0<N>::iters * Pass0<N>::R <=
+              pass0_register_points(lds_waves_per_simd(fixed_lds_bytes<N>(), Fixed<N>::P.threads));
+}
+
+// The work item's index within its transform as used inside one group of
+// the persistent loop. For the one-transform workgroups of N > 4096 it is
+// re-read per group (an empty asm the compiler cannot see through), so that
+// the LDS slots and twiddle offsets of every pass are recomputed per group:
+// hoisted out of the loop they held about 100 registers (N = 16384: 212
+// bytes of spills per work item, whose reloads also waited for the
+// prefetched loads).
+template <int N>
+__device__ __forceinline__ uint32_t group_item(uint32_t t)
+{
+   if constexpr (Fixed<N>::P.threads > kFixedThreads) asm volatile("" : "+v"(t));
+   return t;
+}
+
+template <int N, int KIND>
+__global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs a)
 {
    constexpr FixedPlan P      = Fixed<N>::P;
    constexpr int       T      = P.T;
@@ -853,18 +915,19 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       uint32_t g = blockIdx.x;
       if (g < a.numGroups) pass0_load<N, KIND>(v, g * FPB, fl, min((uint32_t) FPB, a.numChannels - g * FPB), t, a);
       if constexpr (fixed_tw_lds<N>()) {
-         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += P.threads) smem[FPB * STRIDE + i] = a.tw[i];
          __syncthreads();
       }
       for (; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
-         pass0_compute<N>(v, data, t, twp, sc);
+         const uint32_t tg    = group_item<N>(t);
+         pass0_compute<N>(v, data, tg, twp, sc);
          const uint32_t gn = g + gridDim.x;
-         if (gn < a.numGroups) pass0_load<N, KIND>(v, gn * FPB, fl, min((uint32_t) FPB, a.numChannels - gn * FPB), t, a);
+         if (gn < a.numGroups) pass0_load<N, KIND>(v, gn * FPB, fl, min((uint32_t) FPB, a.numChannels - gn * FPB), tg, a);
          __syncthreads();
-         fixed_passes_from<N, 1>(data, t, twp);
-         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
+         fixed_passes_from<N, 1>(data, tg, twp);
+         group_store<N, KIND>(smem, data, first, nf, fl, tg, a, map);
          __syncthreads();
       }
    }
@@ -877,7 +940,7 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
       // (V[i] = X[N+i], no mirror): work item t computes points t + T*it, 8
       // at a time.
       if constexpr (fixed_tw_lds<N>()) {
-         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += P.threads) smem[FPB * STRIDE + i] = a.tw[i];
       }
       constexpr int iters = N / T;
       constexpr int CH    = largest_divisor_upto(iters, 8);
@@ -948,35 +1011,37 @@ __global__ void __launch_bounds__(kFixedThreads) fixed_kernel(FixedArgs a)
    else if constexpr (fixed_direct<N>()) {
       using Q = Pass0<N>;
       if constexpr (fixed_tw_lds<N>()) {
-         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += P.threads) smem[FPB * STRIDE + i] = a.tw[i];
          __syncthreads();
       }
       for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+         const uint32_t tg    = group_item<N>(t);
 #pragma unroll 1
          for (uint32_t u0 = 0; u0 < (uint32_t) Q::items; u0 += T) {
             float2 v[Q::R];
-            pass0_load_one<N, KIND>(v, first, fl, nf, t + u0, a);
-            pass0_compute_one<N>(v, data, t + u0, twp, sc);
+            pass0_load_one<N, KIND>(v, first, fl, nf, tg + u0, a);
+            pass0_compute_one<N>(v, data, tg + u0, twp, sc);
          }
          __syncthreads();
-         fixed_passes_from<N, 1>(data, t, twp);
-         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
+         fixed_passes_from<N, 1>(data, tg, twp);
+         group_store<N, KIND>(smem, data, first, nf, fl, tg, a, map);
          __syncthreads();
       }
    }
    else {
       if constexpr (fixed_tw_lds<N>()) {
-         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += kFixedThreads) smem[FPB * STRIDE + i] = a.tw[i];
+         for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += P.threads) smem[FPB * STRIDE + i] = a.tw[i];
       }
       for (uint32_t g = blockIdx.x; g < a.numGroups; g += gridDim.x) {
          const uint32_t first = g * FPB;
          const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+         const uint32_t tg    = group_item<N>(t);
          stage_group<N, KIND, STRIDE>(a, first, nf, sc, smem);
          __syncthreads();
-         fixed_passes_from<N, 0>(data, t, twp);
-         group_store<N, KIND>(smem, data, first, nf, fl, t, a, map);
+         fixed_passes_from<N, 0>(data, tg, twp);
+         group_store<N, KIND>(smem, data, first, nf, fl, tg, a, map);
          __syncthreads();
       }
    }
@@ -1137,7 +1202,9 @@ __device__ __forceinline__ void rows_store(const float2 *smem, uint32_t first, c
    }
 }
 
-template <int N, int KIND>
+// BUDGET: points per work item the first pass may hold in registers
+// (pass0_register_points of the calling kernel's LDS).
+template <int N, int KIND, int BUDGET>
 __device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, float2 *smem)
 {
    constexpr FixedPlan P      = Fixed<N>::P;
@@ -1156,11 +1223,18 @@ __device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, floa
    const uint32_t      nf     = min((uint32_t) FPB, a.numChannels - first);
    if constexpr (fixed_direct<N>()) {
       using Q = Pass0<N>;
+      if constexpr (Q::iters * Q::R <= BUDGET) {
+         float2 v[Q::iters][Q::R];
+         pass0_load<N, KIND>(v, first, fl, nf, t, a);
+         pass0_compute<N>(v, data, t, a.tw, sc);
+      }
+      else {
 #pragma unroll 1
-      for (uint32_t u0 = 0; u0 < (uint32_t) Q::items; u0 += T) {
-         float2 v[Q::R];
-         pass0_load_one<N, KIND>(v, first, fl, nf, t + u0, a);
-         pass0_compute_one<N>(v, data, t + u0, a.tw, sc);
+         for (uint32_t u0 = 0; u0 < (uint32_t) Q::items; u0 += T) {
+            float2 v[Q::R];
+            pass0_load_one<N, KIND>(v, first, fl, nf, t + u0, a);
+            pass0_compute_one<N>(v, data, t + u0, a.tw, sc);
+         }
       }
       __syncthreads();
       fixed_passes_from<N, 1>(data, t, a.tw);
@@ -1283,7 +1357,7 @@ using ClassTaskK = const __attribute__((address_space(4))) ClassTask;
 // restore the callee-saved registers it uses (up to 50 VGPRs, 20-30% of a
 // group's time); with plain calls from the one kernel the compiler leaves
 // them unsaved and the kernel does not rely on them.
-template <int N>
+template <int N, int BUDGET>
 __device__ __attribute__((noinline, not_tail_called)) void class_group(uint64_t task, uint64_t args, uint32_t g, uint32_t twSlot,
                                                       uint32_t heldSlot)
 {
@@ -1298,33 +1372,44 @@ __device__ __attribute__((noinline, not_tail_called)) void class_group(uint64_t 
       const float2 **held = (const float2 **) (smem_class + heldSlot);
       float2        *twl  = smem_class + twSlot;
       if ((const float2 *) wave_uniform((uint64_t) *held) != a.tw) {   // one LDS word read by every work item
-         for (uint32_t i = threadIdx.x; i < (uint32_t) Fixed<N>::P.twCount; i += kFixedThreads) twl[i] = a.tw[i];
+         for (uint32_t i = threadIdx.x; i < (uint32_t) Fixed<N>::P.twCount; i += Fixed<N>::P.threads) twl[i] = a.tw[i];
          __syncthreads();
          if (threadIdx.x == 0) *held = a.tw;   // read again only after this group's closing barrier
       }
       a.tw = twl;
    }
-   fixed_group<N, KIND_C2C>(a, g, smem_class);
+   fixed_group<N, KIND_C2C, BUDGET>(a, g, smem_class);
 }
 
-// Waves per SIMD the class's LDS allows (one wave of each resident
-// workgroup per SIMD, 160 KB of LDS per CU, at most 4): the register budget
-// of the kernel and of every per-length function it calls follows from it
-// (512 / waves VGPRs).
+// Work items per workgroup of a class kernel: the same for all its lengths
+// (an octave shares T and FPB).
+template <int L0, int... Ls>
+constexpr int class_threads()
+{
+   static_assert(((Fixed<Ls>::P.threads == Fixed<L0>::P.threads) && ...), "a class has one workgroup size");
+   return Fixed<L0>::P.threads;
+}
+
+// Waves per SIMD the class's LDS allows (threads / 256 waves of each
+// resident workgroup per SIMD, 160 KB of LDS per CU), at least one
+// workgroup's and at most 4: the register budget of the kernel and of every
+// per-length function it calls follows from it (512 / waves VGPRs).
 template <int... Ls>
 constexpr int class_waves()
 {
-   const size_t w = (size_t) 163840 / class_lds_bytes<Ls...>();
-   return w < 1 ? 1 : w > 4 ? 4 : (int) w;
+   const int w = lds_waves_per_simd(class_lds_bytes<Ls...>(), class_threads<Ls...>());
+   const int m = class_threads<Ls...>() / 256;
+   return w < m ? m : w > 4 ? (m > 4 ? m : 4) : w;
 }
 
 template <int... Ls>
-__global__ void __launch_bounds__(kFixedThreads) __attribute__((amdgpu_waves_per_eu(class_waves<Ls...>())))
+__global__ void __launch_bounds__(class_threads<Ls...>()) __attribute__((amdgpu_waves_per_eu(class_waves<Ls...>())))
 fixed_class_kernel(ClassArgs c)
 {
    extern __shared__ float2 smem_class[];
    constexpr uint32_t twSlot   = class_data_slots<Ls...>();
    constexpr uint32_t heldSlot = twSlot + class_tw_slots<Ls...>();
+   constexpr int      kBudget  = pass0_register_points(class_waves<Ls...>());
    if (threadIdx.x == 0) *(const float2 **) (smem_class + heldSlot) = nullptr;
    __syncthreads();
    // Items in XCD-contiguous order: workgroups are dispatched round robin over
@@ -1352,7 +1437,7 @@ fixed_class_kernel(ClassArgs c)
          g                   = item & ((1u << kClassGroupBits) - 1u);
          n                   = c.tasks[item >> kClassGroupBits].n;
       }
-      (void) ((n == (uint32_t) Ls ? (class_group<Ls>(task, args, g, twSlot, heldSlot), true) : false) || ...);
+      (void) ((n == (uint32_t) Ls ? (class_group<Ls, kBudget>(task, args, g, twSlot, heldSlot), true) : false) || ...);
    }
 }
 
@@ -1365,7 +1450,7 @@ uint32_t class_resident_groups()
       if (hipGetDevice(&dev) != hipSuccess ||
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
          cus = 256;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fixed_class_kernel<Ls...>, kFixedThreads,
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fixed_class_kernel<Ls...>, class_threads<Ls...>(),
                                                        class_lds_bytes<Ls...>()) != hipSuccess ||
           per <= 0)
          per = 1;
@@ -1394,7 +1479,7 @@ hipError_t launch_class(const ClassArgs &c, uint32_t grid, hipStream_t stream)
    // autotuner's choice) replaces it; never more workgroups than items.
    uint32_t target = (uint32_t) std::min<uint64_t>(c.numItems, 8ull * class_resident_groups<Ls...>());
    if (grid != 0) target = min(c.numItems, grid);
-   hipLaunchKernelGGL((fixed_class_kernel<Ls...>), dim3(target), dim3(kFixedThreads), lds, stream, c);
+   hipLaunchKernelGGL((fixed_class_kernel<Ls...>), dim3(target), dim3(class_threads<Ls...>()), lds, stream, c);
    return hipGetLastError();
 }
 
@@ -1409,7 +1494,7 @@ uint32_t fixed_resident_groups(size_t lds)
       if (hipGetDevice(&dev) != hipSuccess ||
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
          cus = 256;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fixed_kernel<N, KIND>, kFixedThreads, lds) !=
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fixed_kernel<N, KIND>, Fixed<N>::P.threads, lds) !=
               hipSuccess ||
           per <= 0)
          per = 1;
@@ -1471,7 +1556,7 @@ hipError_t launch_fixed(FixedArgs a, hipStream_t stream, uint32_t maxGrid = 0)
       grid               = (uint32_t) std::min<uint64_t>(a.numGroups, std::max<uint64_t>(resident, aim));
    }
    if (maxGrid != 0 && grid > maxGrid) grid = maxGrid;
-   hipLaunchKernelGGL((fixed_kernel<N, KIND>), dim3(grid), dim3(kFixedThreads), lds, stream, a);
+   hipLaunchKernelGGL((fixed_kernel<N, KIND>), dim3(grid), dim3(Fixed<N>::P.threads), lds, stream, a);
    return hipGetLastError();
 }
 

@@ -301,6 +301,24 @@ def test_c2c_variant_other_lengths_use_fp32(c2c_variant):
     assert np.array_equal(y0.view(np.uint32), y1.view(np.uint32))
 
 
+# Lengths between 4096 and 16384 points without a dedicated kernel: the
+# octave kernels, one transform per workgroup of N / 16 work items (512 up to
+# 8192, 1024 beyond); odd lengths included (15625 = 5^6, 6561 = 3^8).
+CLASS_LONG_SIZES = [4320, 5000, 6250, 6561, 7776, 8640, 10000, 12000, 12500, 15625, 16200]
+
+
+@pytest.mark.parametrize("kind", ["fft", "ifft", "ifft_single"])
+@pytest.mark.parametrize("n", CLASS_LONG_SIZES)
+def test_c2c_class_long(n, kind):
+    single = kind == "ifft_single"
+    b = 1 if single else 3
+    x = cplx(rng(n + 3 * len(kind)), b, n, 1.0)
+    s, y = G.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert s == 0, G.F.last_error()
+    ref = oracle.c2c(x, n, b, inverse=kind != "fft", single=single)
+    assert G.rel_l2(y, ref) <= REL_TOL
+
+
 # Complex lengths longer than 16384 points: a column pass and a row pass on
 # the multi-length kernels (n = n1 * m2: 20000 = 160 * 125, 16875 = 135 * 125,
 # 65536 = 256 * 256, 2^20 = 1024 * 1024); with the generic kernels, outer radix
