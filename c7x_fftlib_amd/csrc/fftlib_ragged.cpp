@@ -37,6 +37,7 @@
 #include "fft_fixed_impl.h"
 #include "fft_kernels.h"
 #include "fftlib_gpu.h"
+#include "fftlib_ragged_schedule.h"
 #include "fftlib_runtime.h"
 
 using namespace fftlib;
@@ -155,47 +156,63 @@ void release(FFTLIB_GPU_ragged_s *h)
    if (h->capture) (void) hipStreamDestroy(h->capture);
 }
 
-// Fork from origin to every lane, and join every lane back into origin.
-hipError_t fork_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
+// Phase of each class launch, in the plan's order.
+std::vector<int> phases_of(const FFTLIB_GPU_ragged_s *h)
+{
+   std::vector<int> p;
+   for (const auto &c : h->classes) p.push_back(c.phase);
+   return p;
+}
+
+// Fork from origin to every lane (records event ev), and join every lane back
+// into origin (lane i records event ev + i). The pool holds
+// ragged::events_needed of the plan's two-direction schedule; the index
+// checks turn a schedule that names more events into an error instead of a
+// read past the pool.
+hipError_t fork_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t ev)
 {
    hipError_t e;
-   if (ev >= h->events.size()) return hipErrorInvalidValue;   // pool sized by ragged_event_demand
-   hipEvent_t fork = h->events[ev++];
+   if (ev >= h->events.size()) return hipErrorInvalidValue;
+   hipEvent_t fork = h->events[ev];
    if ((e = hipEventRecord(fork, origin)) != hipSuccess) return e;
    for (hipStream_t lane : h->lanes)
       if ((e = hipStreamWaitEvent(lane, fork, 0)) != hipSuccess) return e;
    return hipSuccess;
 }
-hipError_t join_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t &ev)
+hipError_t join_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t ev)
 {
    hipError_t e;
-   for (hipStream_t lane : h->lanes) {
-      if (ev >= h->events.size()) return hipErrorInvalidValue;
-      hipEvent_t join = h->events[ev++];
-      if ((e = hipEventRecord(join, lane)) != hipSuccess) return e;
+   for (size_t i = 0; i < h->lanes.size(); ++i) {
+      if (ev + i >= h->events.size()) return hipErrorInvalidValue;
+      hipEvent_t join = h->events[ev + i];
+      if ((e = hipEventRecord(join, h->lanes[i])) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(origin, join, 0)) != hipSuccess) return e;
    }
    return hipSuccess;
 }
 
-// Enqueues one direction: the class kernels of phase 0, a join, then those
-// of phase 1, each phase dealt to the lanes largest first.
-hipError_t enqueue_direction(FFTLIB_GPU_ragged_s *h, const float *in, float *out, int mode, hipStream_t origin,
-                             size_t &ev)
+// One direction's buffers and mode.
+struct Direction {
+   const float *in;
+   float       *out;
+   int          mode;
+};
+
+// Enqueues `count` directions by walking their schedule
+// (fftlib_ragged_schedule.h): per direction the class kernels of phase 0, a
+// join, then those of phase 1, each phase dealt to the lanes largest first.
+hipError_t enqueue(FFTLIB_GPU_ragged_s *h, const Direction *dirs, int count, hipStream_t origin)
 {
-   hipError_t e;
-   for (int phase = 0; phase < 2; ++phase) {
-      bool any = false;
-      for (const auto &c : h->classes) any = any || c.phase == phase;
-      if (!any) continue;
-      if ((e = fork_lanes(h, origin, ev)) != hipSuccess) return e;
-      size_t next = 0;
-      for (const auto &c : h->classes) {
-         if (c.phase != phase) continue;
-         if ((e = launch_class(*h, c, in, out, mode, c.grid, h->lanes[next++ % h->lanes.size()])) != hipSuccess)
-            return e;
+   hipError_t e = hipSuccess;
+   for (const ragged::Step &st : ragged::schedule(phases_of(h), (int) h->lanes.size(), count)) {
+      if (st.op == ragged::Op::Fork) e = fork_lanes(h, origin, st.ev);
+      else if (st.op == ragged::Op::Join) e = join_lanes(h, origin, st.ev);
+      else {
+         const auto      &c = h->classes[(size_t) st.cls];
+         const Direction &d = dirs[st.dir];
+         e                  = launch_class(*h, c, d.in, d.out, d.mode, c.grid, h->lanes[(size_t) st.lane]);
       }
-      if ((e = join_lanes(h, origin, ev)) != hipSuccess) return e;
+      if (e != hipSuccess) return e;
    }
    return hipSuccess;
 }
@@ -206,14 +223,8 @@ hipError_t capture_graph(FFTLIB_GPU_ragged_s *h, const float *x, float *y, float
    if (h->graph) (void) hipGraphDestroy(h->graph), h->graph = nullptr;
    hipError_t e = hipStreamBeginCapture(h->capture, hipStreamCaptureModeThreadLocal);
    if (e != hipSuccess) return e;
-   size_t     ev = 0;
-   hipError_t le = hipSuccess;
-   if (dir == 0) le = enqueue_direction(h, x, y, C2C_FORWARD, h->capture, ev);
-   else if (dir == 1) le = enqueue_direction(h, x, y, C2C_INVERSE_PRESCALE, h->capture, ev);
-   else {
-      le = enqueue_direction(h, x, y, C2C_FORWARD, h->capture, ev);
-      if (le == hipSuccess) le = enqueue_direction(h, y, z, C2C_INVERSE_PRESCALE, h->capture, ev);
-   }
+   const Direction dirs[2] = {{x, y, dir == 1 ? C2C_INVERSE_PRESCALE : C2C_FORWARD}, {y, z, C2C_INVERSE_PRESCALE}};
+   const hipError_t le     = enqueue(h, dirs, dir == 2 ? 2 : 1, h->capture);
    hipGraph_t graph = nullptr;
    e                = hipStreamEndCapture(h->capture, &graph);
    if (le != hipSuccess) {
@@ -382,8 +393,8 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       }
       h->lanes.push_back(s);
    }
-   // two fork/join phases for each of up to two directions
-   const size_t nev = 2 * 2 * (1 + h->lanes.size());
+   // the events of the largest call's schedule (forward then inverse)
+   const size_t nev = ragged::events_needed(ragged::schedule(phases_of(h), lanes, 2), lanes);
    for (size_t i = 0; i < nev; ++i) {
       hipEvent_t ev;
       if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
@@ -424,8 +435,8 @@ FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB
       // capture; the warm-up runs after the caller's pending work on pX / pY
       if (handle->exec == nullptr) {
          if ((e = order_after_caller(handle)) != hipSuccess) return fail("ragged stream ordering", e);
-         size_t ev = 0;
-         if ((e = enqueue_direction(handle, pX, pY, C2C_FORWARD, handle->capture, ev)) != hipSuccess)
+         const Direction fwd{pX, pY, C2C_FORWARD};
+         if ((e = enqueue(handle, &fwd, 1, handle->capture)) != hipSuccess)
             return fail("ragged warm-up launch", e);
          if ((e = hipStreamSynchronize(handle->capture)) != hipSuccess) return fail("ragged warm-up", e);
       }
