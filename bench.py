@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--no-variants", action="store_true",
                     help="c2c_960 fp32: skip timing the split-precision MFMA variants after the headline")
     ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target seconds per CPU baseline run")
+    ap.add_argument("--sweep-mfma", action="store_true",
+                    help="instead of the workload: fp32 against the three MFMA variants at every MFMA length, "
+                         "2^25 points per call, one JSON line")
     return ap.parse_args()
 
 
@@ -640,6 +643,60 @@ def time_variants(wl, args, lib, F, torch, stream, fp32_ms):
     return res
 
 
+MFMA_LENGTHS = [32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256, 288, 320, 384, 480, 512, 576, 640, 768, 960, 1024]
+
+
+def sweep_mfma(lib, F, torch, stream):
+    """Per MFMA length: fp32 and the three split variants on 2^25 points per
+    call (best of 3 x 20 launches between HIP events on the launch stream) and
+    each one's rel L2 error on 64 transforms against numpy's FP64 FFT."""
+    points = 1 << 25
+    name = "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc"
+    fn = lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_kernel
+    r = philox(25)
+    rows = []
+    for n in MFMA_LENGTHS:
+        b = points // n
+        host = np.empty((b, n), np.complex64)
+        host.real = r.uniform(-1, 1, (b, n))
+        host.imag = r.uniform(-1, 1, (b, n))
+        x = torch.from_numpy(host.view(np.float32).reshape(-1)).to("cuda:0")
+        y = torch.empty_like(x)
+        w = np.zeros(2 * n, np.float32)
+        lib.FFTLIB_tw_gen_c2c(w.ctypes.data, n)
+        wd = torch.from_numpy(w).to("cuda:0")
+        pb = ctypes.create_string_buffer(F.PBLOCK_SIZE[name])
+        bx, bw = F.BufParams1D(F.FFTLIB_FLOAT32, 2 * n * b), F.BufParams1D(F.FFTLIB_FLOAT32, 2 * n)
+        args = (x.data_ptr(), ctypes.byref(bx), wd.data_ptr(), ctypes.byref(bw), y.data_ptr(), ctypes.byref(bx), n, b,
+                pb)
+        assert lib.FFTLIB_fft1dBatched_i32fc_c32fc_o32fc_init(*args) == 0, F.last_error()
+        ref = np.fft.fft(host[:64].astype(np.complex128), axis=1)
+        row = {"n": n, "channels": b}
+        for vname, code in VARIANT_CODES.items():
+            assert lib.FFTLIB_GPU_setC2CVariant(code) == 0, F.last_error()
+            for _ in range(5):
+                assert fn(*args) == 0, F.last_error()
+            torch.cuda.synchronize()
+            best = float("inf")
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(20):
+                    assert fn(*args) == 0, F.last_error()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1) / 20 * 1e3)
+            got = y[: 2 * n * 64].cpu().numpy().view(np.complex64).reshape(64, n).astype(np.complex128)
+            row[vname] = {"us": round(best, 1), "rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
+        assert lib.FFTLIB_GPU_setC2CVariant(0) == 0
+        rows.append(row)
+        print(f"sweep n={n}: " + ", ".join(f"{k} {row[k]['us']} us" for k in VARIANT_CODES), file=sys.stderr,
+              flush=True)
+        del x, y
+    return {"sweep": "C2C fp32 vs split-precision MFMA variants per length", "points_per_call": points,
+            "timing": "best of 3 x 20 launches, HIP events on the launch stream", "rows": rows}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -662,6 +719,11 @@ def main():
         code = {"fp32": 0, "bf16x3": 1, "fp16x2": 2, "bf16x2": 3}[args.variant]
         assert sel(code) == 0, F.last_error()
 
+    if args.sweep_mfma:
+        lib.FFTLIB_GPU_setSynchronous(0)
+        print(json.dumps(sweep_mfma(lib, F, torch, stream)), flush=True)
+        lib.FFTLIB_GPU_setSynchronous(1)
+        return
     wl = Workload(args, F, lib, torch)
     lib.FFTLIB_GPU_setSynchronous(0)
     ms, wall, ramp_steps, ramp_s = time_steps(wl, args, stream, args.min_warmup_s)
