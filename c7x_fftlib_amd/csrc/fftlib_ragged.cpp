@@ -20,9 +20,9 @@
 // therefore two phases: phase 0 = the lengths up to 16384 and the column
 // tasks, phase 1 = the row tasks, each at most one class-kernel launch per
 // octave spread over several capture streams, with a join between the
-// phases. Each class kernel walks a persistent list of (task, group) items,
-// largest tasks first, so the whole device stays busy with many short
-// transforms of different lengths at once.
+// phases (fftlib_ragged_schedule.h). Each class kernel walks a persistent
+// list of (task, group) items, largest tasks first, so the whole device
+// stays busy with many short transforms of different lengths at once.
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -57,6 +57,7 @@ struct FFTLIB_GPU_ragged_s {
    struct Class {
       int                     cls      = -1;
       int                     phase    = 0;
+      bool                    columns  = false;   // phase 0: holds column tasks of the longer lengths
       uint32_t                numItems = 0;
       fixedk::ClassTask      *dTasks   = nullptr;
       uint32_t               *dItems   = nullptr;
@@ -156,39 +157,13 @@ void release(FFTLIB_GPU_ragged_s *h)
    if (h->capture) (void) hipStreamDestroy(h->capture);
 }
 
-// Phase of each class launch, in the plan's order.
-std::vector<int> phases_of(const FFTLIB_GPU_ragged_s *h)
+// Kind of each class launch, in the plan's order.
+std::vector<ragged::Kind> kinds_of(const FFTLIB_GPU_ragged_s *h)
 {
-   std::vector<int> p;
-   for (const auto &c : h->classes) p.push_back(c.phase);
-   return p;
-}
-
-// Fork from origin to every lane (records event ev), and join every lane back
-// into origin (lane i records event ev + i). The pool holds
-// ragged::events_needed of the plan's two-direction schedule; the index
-// checks turn a schedule that names more events into an error instead of a
-// read past the pool.
-hipError_t fork_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t ev)
-{
-   hipError_t e;
-   if (ev >= h->events.size()) return hipErrorInvalidValue;
-   hipEvent_t fork = h->events[ev];
-   if ((e = hipEventRecord(fork, origin)) != hipSuccess) return e;
-   for (hipStream_t lane : h->lanes)
-      if ((e = hipStreamWaitEvent(lane, fork, 0)) != hipSuccess) return e;
-   return hipSuccess;
-}
-hipError_t join_lanes(FFTLIB_GPU_ragged_s *h, hipStream_t origin, size_t ev)
-{
-   hipError_t e;
-   for (size_t i = 0; i < h->lanes.size(); ++i) {
-      if (ev + i >= h->events.size()) return hipErrorInvalidValue;
-      hipEvent_t join = h->events[ev + i];
-      if ((e = hipEventRecord(join, h->lanes[i])) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(origin, join, 0)) != hipSuccess) return e;
-   }
-   return hipSuccess;
+   std::vector<ragged::Kind> k;
+   for (const auto &c : h->classes)
+      k.push_back(c.phase == 1 ? ragged::Kind::Row : c.columns ? ragged::Kind::Column : ragged::Kind::Plain);
+   return k;
 }
 
 // One direction's buffers and mode.
@@ -198,20 +173,24 @@ struct Direction {
    int          mode;
 };
 
-// Enqueues `count` directions by walking their schedule
-// (fftlib_ragged_schedule.h): per direction the class kernels of phase 0, a
-// join, then those of phase 1, each phase dealt to the lanes largest first.
+// Enqueues `count` directions by performing the steps of their schedule
+// (fftlib_ragged_schedule.h) on the origin stream and the lanes. The pool
+// holds ragged::events_needed of the plan's two-direction schedule; the index
+// check turns a schedule that names more events into an error instead of a
+// read past the pool.
 hipError_t enqueue(FFTLIB_GPU_ragged_s *h, const Direction *dirs, int count, hipStream_t origin)
 {
    hipError_t e = hipSuccess;
-   for (const ragged::Step &st : ragged::schedule(phases_of(h), (int) h->lanes.size(), count)) {
-      if (st.op == ragged::Op::Fork) e = fork_lanes(h, origin, st.ev);
-      else if (st.op == ragged::Op::Join) e = join_lanes(h, origin, st.ev);
-      else {
+   for (const ragged::Step &st : ragged::schedule(kinds_of(h), (int) h->lanes.size(), count)) {
+      const hipStream_t s = st.stream == ragged::kOrigin ? origin : h->lanes[(size_t) st.stream];
+      if (st.op == ragged::Op::Launch) {
          const auto      &c = h->classes[(size_t) st.cls];
          const Direction &d = dirs[st.dir];
-         e                  = launch_class(*h, c, d.in, d.out, d.mode, c.grid, h->lanes[(size_t) st.lane]);
+         e                  = launch_class(*h, c, d.in, d.out, d.mode, c.grid, s);
       }
+      else if (st.ev >= h->events.size()) e = hipErrorInvalidValue;
+      else if (st.op == ragged::Op::Record) e = hipEventRecord(h->events[st.ev], s);
+      else e = hipStreamWaitEvent(s, h->events[st.ev], 0);
       if (e != hipSuccess) return e;
    }
    return hipSuccess;
@@ -283,6 +262,7 @@ hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
       c.phase = kv.first.first;
       c.cls   = kv.first.second;
       c.work  = work[kv.first];
+      for (const auto &t : kv.second) c.columns = c.columns || t.layout == fixedk::LAYOUT_COLUMNS;
       std::vector<uint32_t> items;
       for (uint32_t t = 0; t < kv.second.size(); ++t) {
          if (kv.second[t].numGroups >= (1u << fixedk::kClassGroupBits) || t >= 4096u) return hipErrorInvalidValue;
@@ -394,7 +374,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       h->lanes.push_back(s);
    }
    // the events of the largest call's schedule (forward then inverse)
-   const size_t nev = ragged::events_needed(ragged::schedule(phases_of(h), lanes, 2), lanes);
+   const size_t nev = ragged::events_needed(ragged::schedule(kinds_of(h), lanes, 2));
    for (size_t i = 0; i < nev; ++i) {
       hipEvent_t ev;
       if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {

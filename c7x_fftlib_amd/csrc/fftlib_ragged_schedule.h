@@ -1,13 +1,26 @@
 // Capture schedule of a ragged plan, as plain host code (no HIP types) so
-// that its event accounting can be checked on a machine without a GPU.
+// that its event accounting and launch ordering can be checked on a machine
+// without a GPU.
 //
-// Each direction is, for each phase that has launches: a fork (one event
-// recorded on the origin stream and waited on by every lane), the phase's
-// class launches dealt round robin over the lanes, and a join (one event per
-// lane, recorded on the lane and waited on by the origin). Every fork and
-// join step names the pool events it records, each event once per schedule;
-// the plan sizes its pool with events_needed of its largest call (forward
-// then inverse) and the capture records exactly the events the steps name.
+// A schedule is a list of primitive steps on the origin stream and the
+// capture lanes: record an event, make a stream wait for an event, launch a
+// class kernel. Every event of the pool is recorded at most once per
+// schedule; the plan sizes its pool with events_needed of its largest call
+// (forward then inverse) and the capture performs exactly these steps.
+//
+// Class launches are of three kinds: plain (lengths up to 16384 only),
+// column (phase 0 launches that hold the column tasks of the longer lengths,
+// possibly with plain tasks too) and row (their row tasks). Row launches read
+// what the column launches wrote, so they must follow every column launch of
+// their direction. Each direction: a fork (origin records, every lane
+// waits), phase 0 (plain and column launches) round robin, a join (every
+// lane records, the origin waits); then the same for phase 1 (row launches).
+// The next direction starts after the last join, so it sees every write of
+// the previous one (the inverse reads the forward's output and reuses the
+// scratch region). Graphs whose row launches waited only for the column
+// launches (lanes waiting on other lanes' events while further lanes still
+// held unjoined work) crashed the host process inside the graph capture or
+// launch; the schedule keeps every wait at a fork or a join.
 #pragma once
 
 #include <stddef.h>
@@ -18,47 +31,73 @@
 namespace fftlib {
 namespace ragged {
 
-enum class Op : uint8_t { Fork, Launch, Join };
+enum class Kind : uint8_t { Plain, Column, Row };
+enum class Op : uint8_t { Record, Wait, Launch };
+
+// Stream of a step: a lane index, or kOrigin for the stream the call is
+// captured on.
+constexpr int kOrigin = -1;
 
 struct Step {
    Op     op;
-   int    dir;    // direction of the step (0 = the first one captured)
-   int    cls;    // Launch: index of the class launch in the plan
-   int    lane;   // Launch: capture stream it is enqueued on
-   size_t ev;     // Fork: its event; Join: the first of `lanes` events (lane i records ev + i)
+   int    dir;      // direction of the step (0 = the first one captured)
+   int    stream;   // lane or kOrigin
+   int    cls;      // Launch: index of the class launch in the plan
+   size_t ev;       // Record / Wait: pool event
 };
 
-// phases[i]: phase (0 or 1) of class launch i; directions: 1 or 2.
-inline std::vector<Step> schedule(const std::vector<int> &phases, int lanes, int directions)
-{
+namespace detail {
+
+struct Builder {
    std::vector<Step> s;
-   if (lanes < 1) return s;
-   size_t ev = 0;
+   size_t            ev = 0;
+   int               dir = 0;
+   size_t record(int stream)
+   {
+      s.push_back({Op::Record, dir, stream, -1, ev});
+      return ev++;
+   }
+   void wait(int stream, size_t e) { s.push_back({Op::Wait, dir, stream, -1, e}); }
+   void launch(int stream, int cls) { s.push_back({Op::Launch, dir, stream, cls, 0}); }
+   void fork(int lanes)
+   {
+      const size_t e = record(kOrigin);
+      for (int l = 0; l < lanes; ++l) wait(l, e);
+   }
+   void join(int lanes)
+   {
+      for (int l = 0; l < lanes; ++l) wait(kOrigin, record(l));
+   }
+};
+
+}  // namespace detail
+
+// kinds[i]: kind of class launch i; directions: 1 or 2.
+inline std::vector<Step> schedule(const std::vector<Kind> &kinds, int lanes, int directions)
+{
+   detail::Builder b;
+   if (lanes < 1) return b.s;
    for (int d = 0; d < directions; ++d) {
+      b.dir = d;
       for (int phase = 0; phase < 2; ++phase) {
-         bool any = false;
-         for (int p : phases) any = any || p == phase;
-         if (!any) continue;
-         s.push_back({Op::Fork, d, -1, -1, ev});
-         ev += 1;
-         int next = 0;
-         for (int i = 0; i < (int) phases.size(); ++i)
-            if (phases[i] == phase) s.push_back({Op::Launch, d, i, next++ % lanes, 0});
-         s.push_back({Op::Join, d, -1, -1, ev});
-         ev += (size_t) lanes;
+         std::vector<int> ls;
+         for (int i = 0; i < (int) kinds.size(); ++i)
+            if ((kinds[i] == Kind::Row) == (phase == 1)) ls.push_back(i);
+         if (ls.empty()) continue;
+         b.fork(lanes);
+         for (size_t i = 0; i < ls.size(); ++i) b.launch((int) (i % (size_t) lanes), ls[i]);
+         b.join(lanes);
       }
    }
-   return s;
+   return b.s;
 }
 
 // Size of the event pool a schedule needs: one past the last event it names.
-inline size_t events_needed(const std::vector<Step> &s, int lanes)
+inline size_t events_needed(const std::vector<Step> &s)
 {
    size_t n = 0;
-   for (const Step &st : s) {
-      const size_t end = st.op == Op::Fork ? st.ev + 1u : st.op == Op::Join ? st.ev + (size_t) lanes : 0u;
-      n                = end > n ? end : n;
-   }
+   for (const Step &st : s)
+      if (st.op != Op::Launch && st.ev + 1u > n) n = st.ev + 1u;
    return n;
 }
 
