@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "fft_fixed_class.h"
@@ -58,6 +59,7 @@ struct FFTLIB_GPU_ragged_s {
       int                     cls      = -1;
       int                     phase    = 0;
       bool                    columns  = false;   // phase 0: holds column tasks of the longer lengths
+      int                     chain    = -1;      // phase 1: octave of the column tasks of its transforms
       uint32_t                numItems = 0;
       fixedk::ClassTask      *dTasks   = nullptr;
       uint32_t               *dItems   = nullptr;
@@ -92,11 +94,15 @@ bool smooth235(uint32_t n)
 }
 
 // ---- tuning cache -------------------------------------------------------------
-// Key: the class-kernel launch (kClassKeyBase + 100 * phase + octave index)
-// and a power-of-two bucket of its item count; value: the launch's number of
+// Key: the class-kernel launch (kClassKeyBase + 100 * phase + octave index,
+// plus 1000 * (1 + octave of the column tasks) for row launches) and a
+// power-of-two bucket of its item count; value: the launch's number of
 // workgroups (0 = the class kernels' default) and its time.
 constexpr uint32_t kClassKeyBase = 100000u;
-uint32_t class_key(const FFTLIB_GPU_ragged_s::Class &c) { return kClassKeyBase + 100u * (uint32_t) c.phase + (uint32_t) c.cls; }
+uint32_t class_key(const FFTLIB_GPU_ragged_s::Class &c)
+{
+   return kClassKeyBase + 100u * (uint32_t) c.phase + (uint32_t) c.cls + (c.chain >= 0 ? 1000u * (uint32_t) (c.chain + 1) : 0u);
+}
 struct TuneKey {
    uint32_t n, bucket;
    bool     operator<(const TuneKey &o) const { return n != o.n ? n < o.n : bucket < o.bucket; }
@@ -157,13 +163,25 @@ void release(FFTLIB_GPU_ragged_s *h)
    if (h->capture) (void) hipStreamDestroy(h->capture);
 }
 
-// Kind of each class launch, in the plan's order.
+// Kind of each class launch, in the plan's order, and for each row launch
+// the index of the column launch holding its transforms' column tasks.
 std::vector<ragged::Kind> kinds_of(const FFTLIB_GPU_ragged_s *h)
 {
    std::vector<ragged::Kind> k;
    for (const auto &c : h->classes)
       k.push_back(c.phase == 1 ? ragged::Kind::Row : c.columns ? ragged::Kind::Column : ragged::Kind::Plain);
    return k;
+}
+std::vector<int> links_of(const FFTLIB_GPU_ragged_s *h)
+{
+   std::vector<int> l(h->classes.size(), -1);
+   for (size_t i = 0; i < h->classes.size(); ++i) {
+      if (h->classes[i].phase != 1) continue;
+      for (size_t j = 0; j < h->classes.size(); ++j)
+         if (h->classes[j].phase == 0 && h->classes[j].columns && h->classes[j].cls == h->classes[i].chain)
+            l[i] = (int) j;
+   }
+   return l;
 }
 
 // One direction's buffers and mode.
@@ -181,7 +199,7 @@ struct Direction {
 hipError_t enqueue(FFTLIB_GPU_ragged_s *h, const Direction *dirs, int count, hipStream_t origin)
 {
    hipError_t e = hipSuccess;
-   for (const ragged::Step &st : ragged::schedule(kinds_of(h), (int) h->lanes.size(), count)) {
+   for (const ragged::Step &st : ragged::schedule(kinds_of(h), links_of(h), (int) h->lanes.size(), count)) {
       const hipStream_t s = st.stream == ragged::kOrigin ? origin : h->lanes[(size_t) st.stream];
       if (st.op == ragged::Op::Launch) {
          const auto      &c = h->classes[(size_t) st.cls];
@@ -235,10 +253,13 @@ hipError_t order_after_caller(FFTLIB_GPU_ragged_s *h)
 // (task << 20 | group) largest task first.
 hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
 {
-   std::map<std::pair<int, int>, std::vector<fixedk::ClassTask>> tasks;   // (phase, class)
-   std::map<std::pair<int, int>, double>                         work;
+   // (phase, class, chain): row tasks are grouped by the octave of their
+   // transforms' column tasks (chain), -1 for phase 0
+   std::map<std::tuple<int, int, int>, std::vector<fixedk::ClassTask>> tasks;
+   std::map<std::tuple<int, int, int>, double>                         work;
    for (const auto &g : h->groups) {
       std::vector<std::pair<int, fixedk::ClassTask>> ks;
+      int                                             chain = -1;
       if (!g.twoPass) {
          fixedk::ClassTask k{};
          k.n = g.n, k.numChannels = g.count, k.layout = fixedk::LAYOUT_PLAIN, k.tw = (const float2 *) g.dTw;
@@ -249,18 +270,21 @@ hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
       else {
          ks.push_back({0, two_pass_task(g.plan, fixedk::LAYOUT_COLUMNS, g.count, g.dOff, g.scrOff)});
          ks.push_back({1, two_pass_task(g.plan, fixedk::LAYOUT_ROWS, g.count, g.dOff, g.scrOff)});
+         chain = fixed_class_of(ks[0].second.n);
       }
       for (const auto &pk : ks) {
          const int cls = fixed_class_of(pk.second.n);
          if (cls < 0) return hipErrorInvalidValue;
-         tasks[{pk.first, cls}].push_back(pk.second);
-         work[{pk.first, cls}] += g.work / (double) ks.size();
+         const auto key = std::make_tuple(pk.first, cls, pk.first == 1 ? chain : -1);
+         tasks[key].push_back(pk.second);
+         work[key] += g.work / (double) ks.size();
       }
    }
    for (auto &kv : tasks) {
       FFTLIB_GPU_ragged_s::Class c;
-      c.phase = kv.first.first;
-      c.cls   = kv.first.second;
+      c.phase = std::get<0>(kv.first);
+      c.cls   = std::get<1>(kv.first);
+      c.chain = std::get<2>(kv.first);
       c.work  = work[kv.first];
       for (const auto &t : kv.second) c.columns = c.columns || t.layout == fixedk::LAYOUT_COLUMNS;
       std::vector<uint32_t> items;
@@ -374,7 +398,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       h->lanes.push_back(s);
    }
    // the events of the largest call's schedule (forward then inverse)
-   const size_t nev = ragged::events_needed(ragged::schedule(kinds_of(h), lanes, 2));
+   const size_t nev = ragged::events_needed(ragged::schedule(kinds_of(h), links_of(h), lanes, 2));
    for (size_t i = 0; i < nev; ++i) {
       hipEvent_t ev;
       if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {

@@ -10,17 +10,28 @@
 //
 // Class launches are of three kinds: plain (lengths up to 16384 only),
 // column (phase 0 launches that hold the column tasks of the longer lengths,
-// possibly with plain tasks too) and row (their row tasks). Row launches read
-// what the column launches wrote, so they must follow every column launch of
-// their direction. Each direction: a fork (origin records, every lane
-// waits), phase 0 (plain and column launches) round robin, a join (every
-// lane records, the origin waits); then the same for phase 1 (row launches).
+// possibly with plain tasks too) and row. Row tasks are grouped by the
+// column launch that holds the column tasks of their transforms (`links`),
+// so a row launch reads only what its own column launch wrote. Each
+// direction is one of two forms, and every wait in either is at a fork (the
+// origin records, every lane waits) or a join (each lane records, the origin
+// waits):
+//
+//   chained (at least one column launch, fewer column launches than lanes,
+//   every row launch linked to a column launch): a fork; column launch j on
+//   lane j, followed on that lane by the row launches linked to it (stream
+//   order is their dependency); the plain launches round robin on the other
+//   lanes; a join. The row passes run beside the plain launches' tails.
+//
+//   phased (otherwise): a fork, the plain and column launches round robin,
+//   a join; a fork, the row launches round robin, a join.
+//
 // The next direction starts after the last join, so it sees every write of
 // the previous one (the inverse reads the forward's output and reuses the
-// scratch region). Graphs whose row launches waited only for the column
-// launches (lanes waiting on other lanes' events while further lanes still
-// held unjoined work) crashed the host process inside the graph capture or
-// launch; the schedule keeps every wait at a fork or a join.
+// scratch region). Graphs in which lanes waited on other lanes' events
+// mid-capture, while further lanes still held unjoined work, crashed the
+// host process inside the graph capture or launch; no schedule here has such
+// a wait.
 #pragma once
 
 #include <stddef.h>
@@ -72,20 +83,45 @@ struct Builder {
 
 }  // namespace detail
 
-// kinds[i]: kind of class launch i; directions: 1 or 2.
-inline std::vector<Step> schedule(const std::vector<Kind> &kinds, int lanes, int directions)
+// kinds[i]: kind of class launch i; links[i]: for a row launch, the index
+// of its column launch (else ignored); directions: 1 or 2.
+inline std::vector<Step> schedule(const std::vector<Kind> &kinds, const std::vector<int> &links, int lanes,
+                                  int directions)
 {
    detail::Builder b;
    if (lanes < 1) return b.s;
+   const int        n = (int) kinds.size();
+   std::vector<int> columnLane(n, -1), plain;
+   int              nc = 0;
+   for (int i = 0; i < n; ++i) {
+      if (kinds[i] == Kind::Column) columnLane[i] = nc++;
+      if (kinds[i] == Kind::Plain) plain.push_back(i);
+   }
+   bool chained = nc >= 1 && nc < lanes && (int) links.size() == n;
+   for (int i = 0; chained && i < n; ++i)
+      if (kinds[i] == Kind::Row) chained = links[i] >= 0 && links[i] < n && kinds[links[i]] == Kind::Column;
    for (int d = 0; d < directions; ++d) {
       b.dir = d;
+      if (n == 0) continue;
+      if (chained) {
+         b.fork(lanes);
+         for (int i = 0; i < n; ++i) {
+            if (kinds[i] != Kind::Column) continue;
+            b.launch(columnLane[i], i);
+            for (int r = 0; r < n; ++r)
+               if (kinds[r] == Kind::Row && links[r] == i) b.launch(columnLane[i], r);
+         }
+         for (size_t k = 0; k < plain.size(); ++k) b.launch(nc + (int) (k % (size_t) (lanes - nc)), plain[k]);
+         b.join(lanes);
+         continue;
+      }
       for (int phase = 0; phase < 2; ++phase) {
          std::vector<int> ls;
-         for (int i = 0; i < (int) kinds.size(); ++i)
+         for (int i = 0; i < n; ++i)
             if ((kinds[i] == Kind::Row) == (phase == 1)) ls.push_back(i);
          if (ls.empty()) continue;
          b.fork(lanes);
-         for (size_t i = 0; i < ls.size(); ++i) b.launch((int) (i % (size_t) lanes), ls[i]);
+         for (size_t k = 0; k < ls.size(); ++k) b.launch((int) (k % (size_t) lanes), ls[k]);
          b.join(lanes);
       }
    }

@@ -1,7 +1,8 @@
 // CPU check of the ragged plan's capture schedule (fftlib_ragged_schedule.h),
 // the steps the library performs when it captures a ragged call into a
-// graph. For many plans (plain, column and row launches in any mix, 1..9
-// lanes, one or two directions) it checks:
+// graph. For many plans (plain, column and row launches in any mix, each
+// row launch linked to a column launch or not, 1..9 lanes, one or two
+// directions) it checks:
 //  - events: each recorded at most once, every wait on an event recorded
 //    earlier in the schedule, all below events_needed, and the schedule fits
 //    the pool sized for two directions (the library's pool);
@@ -13,8 +14,12 @@
 //    lane-to-lane waits while other lanes held unjoined work crashed the
 //    host inside capture or launch);
 //  - ordering, from a happens-before simulation of record / wait and stream
-//    order: each row launch after every column launch of its direction,
-//    every launch of the second direction after every launch of the first.
+//    order: each row launch after its linked column launch (after every
+//    column launch when unlinked), every launch of the second direction
+//    after every launch of the first; in the chained form (some column
+//    launches, fewer than lanes, every row launch linked) no row launch
+//    after a plain launch of its direction, and in the phased form every
+//    row launch after every launch of phase 0.
 // Prints "ok" and exits 0 on success; prints the failing plans and exits 1.
 #include <stdio.h>
 #include <stdlib.h>
@@ -38,11 +43,11 @@ void fail(const char *what, const std::vector<Kind> &kinds, int lanes, int dirs)
    printf("\n");
 }
 
-void check(const std::vector<Kind> &kinds, int lanes, int dirs)
+void check(const std::vector<Kind> &kinds, const std::vector<int> &links, int lanes, int dirs)
 {
-   const std::vector<Step> s     = schedule(kinds, lanes, dirs);
+   const std::vector<Step> s     = schedule(kinds, links, lanes, dirs);
    const size_t            need  = events_needed(s);
-   const size_t            pool2 = events_needed(schedule(kinds, lanes, 2));
+   const size_t            pool2 = events_needed(schedule(kinds, links, lanes, 2));
    const int               nc    = (int) kinds.size();
    if (need > pool2) fail("schedule needs more events than the two-direction pool", kinds, lanes, dirs);
 
@@ -116,6 +121,13 @@ void check(const std::vector<Kind> &kinds, int lanes, int dirs)
          f.insert(id);
       }
    }
+   int  ncol   = 0;
+   bool linked = true;
+   for (int c = 0; c < nc; ++c) {
+      ncol += kinds[c] == Kind::Column;
+      if (kinds[c] == Kind::Row) linked = linked && links[c] >= 0 && links[c] < nc && kinds[links[c]] == Kind::Column;
+   }
+   const bool chained = ncol >= 1 && ncol < lanes && linked;
    for (int d = 0; d < dirs; ++d) {
       for (int c = 0; c < nc; ++c) {
          const int id = d * nc + c;
@@ -125,12 +137,27 @@ void check(const std::vector<Kind> &kinds, int lanes, int dirs)
                const bool is  = before[id].count(id2) != 0;
                if (d2 < d && !is) fail("launch not after the previous direction", kinds, lanes, dirs);
                if (d2 > d && is) fail("launch after the next direction", kinds, lanes, dirs);
-               if (d2 == d && kinds[c] == Kind::Row && kinds[c2] == Kind::Column && !is)
-                  fail("row launch not after a column launch", kinds, lanes, dirs);
+               if (d2 != d || kinds[c] != Kind::Row) continue;
+               const bool myColumn = chained ? links[c] == c2 : kinds[c2] == Kind::Column;
+               if (myColumn && !is) fail("row launch not after its column launch", kinds, lanes, dirs);
+               if (chained && kinds[c2] == Kind::Plain && is) fail("row launch waits for a plain launch", kinds, lanes, dirs);
+               if (!chained && kinds[c2] != Kind::Row && !is) fail("row launch not after phase 0", kinds, lanes, dirs);
             }
          }
       }
    }
+}
+
+// Links of the row launches: the first column launch, a random one, or -1
+// (no link) when there is no column launch or `unlink`.
+std::vector<int> make_links(const std::vector<Kind> &kinds, bool unlink)
+{
+   std::vector<int> cols, links(kinds.size(), -1);
+   for (int i = 0; i < (int) kinds.size(); ++i)
+      if (kinds[i] == Kind::Column) cols.push_back(i);
+   for (int i = 0; i < (int) kinds.size(); ++i)
+      if (kinds[i] == Kind::Row && !cols.empty() && !unlink) links[i] = cols[(size_t) rand() % cols.size()];
+   return links;
 }
 
 }  // namespace
@@ -148,16 +175,17 @@ int main()
                   for (int i = 0; i < ncol; ++i) k.push_back(Kind::Column);
                   for (int i = 0; i < np; ++i) k.push_back(Kind::Plain);
                   for (int i = 0; i < nr; ++i) k.push_back(Kind::Row);
-                  check(k, lanes, dirs);
+                  check(k, make_links(k, false), lanes, dirs);
+                  check(k, make_links(k, true), lanes, dirs);
                }
    srand(7);
-   for (int t = 0; t < 400; ++t) {
+   for (int t = 0; t < 600; ++t) {
       std::vector<Kind> k;
       const int         n = rand() % 14;
       for (int i = 0; i < n; ++i) k.push_back((Kind) (rand() % 3));
-      check(k, 1 + rand() % 9, 1 + rand() % 2);
+      check(k, make_links(k, rand() % 5 == 0), 1 + rand() % 9, 1 + rand() % 2);
    }
-   if (events_needed(schedule({}, 8, 2)) != 0) fail("empty plan needs events", {}, 8, 2);
+   if (events_needed(schedule({}, {}, 8, 2)) != 0) fail("empty plan needs events", {}, 8, 2);
    if (failures) return 1;
    printf("ok\n");
    return 0;

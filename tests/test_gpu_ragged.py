@@ -135,6 +135,6 @@ def test_ragged_tuning_keeps_results(tmp_path):
     keys = {int(line.split()[0]) for line in text.strip().splitlines()[1:]}
     # phase 0: octave kernels of 250 and of 40000's columns (200; both in
     # (128, 256]), 960 (512, 1024], 4096 (2048, 4096]; phase 1: 40000's rows
-    # (200)
-    assert {100000 + 4, 100000 + 6, 100000 + 8, 100000 + 100 + 4} <= keys
+    # (200), whose column tasks are in octave 4 (key + 1000 * (4 + 1))
+    assert {100000 + 4, 100000 + 6, 100000 + 8, 100000 + 100 + 4 + 5000} <= keys
     rg.close()
