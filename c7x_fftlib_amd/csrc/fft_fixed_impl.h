@@ -1102,8 +1102,10 @@ constexpr uint32_t class_stride()
 // rows have scrStride points. Channels number scrStride columns per
 // transform so that a group never straddles two transforms; columns from
 // span on are idle (they read column span-1 and store nothing).
-template <int N>
-__device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, float2 *smem)
+// TW_LDS: the column tables are the LDS copy at colLds (low table, then the
+// high table at 2^colBits), else a.colTw / a.colTwHi in global memory.
+template <int N, bool TW_LDS>
+__device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, float2 *smem, const float2 *colLds)
 {
    constexpr FixedPlan P      = Fixed<N>::P;
    constexpr int       T      = P.T;
@@ -1148,7 +1150,15 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
             const uint32_t k = min(i0 + (uint32_t) ((c0 + c) * T), (uint32_t) N - 1u);
             v[c]             = col[fslot(fixed_position<N>(k))];
             const uint32_t e = p * k;   // < fullN
-            const float2   h = a.colTwHi[e >> a.colBits], l = a.colTw[e & ((1u << a.colBits) - 1u)];
+            float2         h, l;
+            if constexpr (TW_LDS) {
+               h = colLds[(1u << a.colBits) + (e >> a.colBits)];
+               l = colLds[e & ((1u << a.colBits) - 1u)];
+            }
+            else {
+               h = a.colTwHi[e >> a.colBits];
+               l = a.colTw[e & ((1u << a.colBits) - 1u)];
+            }
             w[c]             = make_float2(h.x * l.x - h.y * l.y, h.x * l.y + h.y * l.x);
          }
 #pragma unroll
@@ -1205,14 +1215,15 @@ __device__ __forceinline__ void rows_store(const float2 *smem, uint32_t first, c
 // BUDGET: points per work item the first pass may hold in registers
 // (pass0_register_points of the calling kernel's LDS).
 template <int N, int KIND, int BUDGET>
-__device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, float2 *smem)
+__device__ __forceinline__ void fixed_group(const FixedArgs &a, uint32_t g, float2 *smem, const float2 *colLds)
 {
    constexpr FixedPlan P      = Fixed<N>::P;
    constexpr int       T      = P.T;
    constexpr int       FPB    = P.FPB;
    constexpr uint32_t  STRIDE = class_stride<N>();
    if (a.layout == LAYOUT_COLUMNS) {
-      column_group<N>(a, g, smem);
+      if (colLds) column_group<N, true>(a, g, smem, colLds);
+      else column_group<N, false>(a, g, smem, nullptr);
       return;
    }
    const uint32_t      fl     = threadIdx.x / T;
@@ -1316,7 +1327,9 @@ constexpr size_t class_length_lds()
 
 // LDS of a class kernel, in float2 slots: the data region of its largest
 // length, then the twiddle region of its largest staged table, then one
-// slot holding the address of the table staged there.
+// slot holding the address of the table staged there; then, where it costs
+// no resident workgroup, the column-pass twiddle tables (kColTwSlots) and
+// one slot holding their address.
 template <int... Ls>
 constexpr uint32_t class_data_slots()
 {
@@ -1333,10 +1346,32 @@ constexpr uint32_t class_tw_slots()
    return m;
 }
 
+// The two column twiddle tables of a two-pass length (W_fullN^e for e <
+// 2^colBits, then W_fullN^(h 2^colBits) for h < fullN / 2^colBits) held in
+// LDS for the lengths up to 65536 (colBits = 8: at most 512 entries): read
+// from L2 they were two gathers per output point over up to 64 lines per
+// wavefront (N = 65536 = 256 x 256, 2^27 points: column pass 515 us against
+// 390 us for the row pass).
+constexpr uint32_t kColTwSlots = 512;
+
+template <int... Ls>
+constexpr size_t class_lds_base_bytes()
+{
+   return ((size_t) class_data_slots<Ls...>() + class_tw_slots<Ls...>() + 1u) * sizeof(float2);
+}
+
+template <int... Ls>
+constexpr bool class_coltw_lds()
+{
+   constexpr size_t base = class_lds_base_bytes<Ls...>();
+   constexpr size_t with = base + (kColTwSlots + 1u) * sizeof(float2);
+   return (size_t) 163840 / with >= 1u && (size_t) 163840 / with == (size_t) 163840 / base;
+}
+
 template <int... Ls>
 constexpr size_t class_lds_bytes()
 {
-   return ((size_t) class_data_slots<Ls...>() + class_tw_slots<Ls...>() + 1u) * sizeof(float2);
+   return class_lds_base_bytes<Ls...>() + (class_coltw_lds<Ls...>() ? (kColTwSlots + 1u) * sizeof(float2) : 0u);
 }
 
 // Arguments of a called function arrive in vector registers; the class
@@ -1357,9 +1392,10 @@ using ClassTaskK = const __attribute__((address_space(4))) ClassTask;
 // restore the callee-saved registers it uses (up to 50 VGPRs, 20-30% of a
 // group's time); with plain calls from the one kernel the compiler leaves
 // them unsaved and the kernel does not rely on them.
+// colSlot: first slot of the kernel's column-table region (0: none).
 template <int N, int BUDGET>
 __device__ __attribute__((noinline, not_tail_called)) void class_group(uint64_t task, uint64_t args, uint32_t g, uint32_t twSlot,
-                                                      uint32_t heldSlot)
+                                                      uint32_t heldSlot, uint32_t colSlot)
 {
    extern __shared__ float2 smem_class[];
    ClassTaskK *tk = (ClassTaskK *) wave_uniform(task);
@@ -1367,7 +1403,23 @@ __device__ __attribute__((noinline, not_tail_called)) void class_group(uint64_t 
    g              = wave_uniform(g);
    twSlot         = wave_uniform(twSlot);
    heldSlot       = wave_uniform(heldSlot);
+   colSlot        = wave_uniform(colSlot);
    FixedArgs a    = class_fixed_args(*c, *tk);
+   const float2 *colLds = nullptr;
+   if (colSlot != 0u && a.layout == LAYOUT_COLUMNS) {
+      const uint32_t L = 1u << a.colBits, nh = (a.fullN + L - 1u) >> a.colBits;
+      if (L + nh <= kColTwSlots) {
+         const float2 **held = (const float2 **) (smem_class + colSlot + kColTwSlots);
+         float2        *ct   = smem_class + colSlot;
+         if ((const float2 *) wave_uniform((uint64_t) *held) != a.colTw) {   // staged once per task
+            for (uint32_t i = threadIdx.x; i < L; i += Fixed<N>::P.threads) ct[i] = a.colTw[i];
+            for (uint32_t i = threadIdx.x; i < nh; i += Fixed<N>::P.threads) ct[L + i] = a.colTwHi[i];
+            __syncthreads();
+            if (threadIdx.x == 0) *held = a.colTw;   // read again only after this group's closing barrier
+         }
+         colLds = ct;
+      }
+   }
    if constexpr (class_tw_lds<N>()) {
       const float2 **held = (const float2 **) (smem_class + heldSlot);
       float2        *twl  = smem_class + twSlot;
@@ -1378,7 +1430,7 @@ __device__ __attribute__((noinline, not_tail_called)) void class_group(uint64_t 
       }
       a.tw = twl;
    }
-   fixed_group<N, KIND_C2C, BUDGET>(a, g, smem_class);
+   fixed_group<N, KIND_C2C, BUDGET>(a, g, smem_class, colLds);
 }
 
 // Work items per workgroup of a class kernel: the same for all its lengths
@@ -1409,8 +1461,12 @@ fixed_class_kernel(ClassArgs c)
    extern __shared__ float2 smem_class[];
    constexpr uint32_t twSlot   = class_data_slots<Ls...>();
    constexpr uint32_t heldSlot = twSlot + class_tw_slots<Ls...>();
+   constexpr uint32_t colSlot  = class_coltw_lds<Ls...>() ? heldSlot + 1u : 0u;
    constexpr int      kBudget  = pass0_register_points(class_waves<Ls...>());
-   if (threadIdx.x == 0) *(const float2 **) (smem_class + heldSlot) = nullptr;
+   if (threadIdx.x == 0) {
+      *(const float2 **) (smem_class + heldSlot) = nullptr;
+      if constexpr (colSlot != 0u) *(const float2 **) (smem_class + colSlot + kColTwSlots) = nullptr;
+   }
    __syncthreads();
    // Items in XCD-contiguous order: workgroups are dispatched round robin over
    // the kXcds dies (workgroup w on die w % kXcds), so the items are dealt in
@@ -1437,7 +1493,8 @@ fixed_class_kernel(ClassArgs c)
          g                   = item & ((1u << kClassGroupBits) - 1u);
          n                   = c.tasks[item >> kClassGroupBits].n;
       }
-      (void) ((n == (uint32_t) Ls ? (class_group<Ls, kBudget>(task, args, g, twSlot, heldSlot), true) : false) || ...);
+      (void) ((n == (uint32_t) Ls ? (class_group<Ls, kBudget>(task, args, g, twSlot, heldSlot, colSlot), true) : false) ||
+              ...);
    }
 }
 
