@@ -548,6 +548,69 @@ __device__ __forceinline__ void r2c_store_group(const float2 *smem, uint32_t fir
    }
 }
 
+// The split factors A[i], B[2N-1-i] of the outputs i = w + 256c a work item
+// stores in r2c_store_group, loaded once per workgroup by the prefetching
+// kernels: loaded per group they were the youngest vector-memory loads of
+// the store phase, and waiting for them (the counter completes in order)
+// also waited for the next group's prefetched input. The LDS slots are
+// recomputed per group (held too, N = 2048 took 177 VGPRs, one wave per
+// SIMD below what its LDS allows).
+template <int N>
+struct R2CSplitCache {
+   static constexpr int iters = (N + kFixedThreads - 1) / kFixedThreads;
+   float2               A[iters], B[iters];
+
+   __device__ __forceinline__ void init(const FixedArgs &a)
+   {
+#pragma unroll
+      for (int c = 0; c < iters; ++c) {
+         const uint32_t i = min(threadIdx.x + (uint32_t) (c * kFixedThreads), (uint32_t) N - 1u);
+         A[c]             = split_factor(a, i);
+         B[c]             = split_factor(a, 2u * N - 1u - i);
+      }
+   }
+};
+
+// r2c_store_group with the factors and slots of an R2CSplitCache; strided
+// outputs (the dedicated kernels have no offset tables).
+template <int N>
+__device__ __forceinline__ void r2c_store_group_cached(const float2 *smem, uint32_t first, uint32_t nf,
+                                                       const FixedArgs &a, const R2CSplitCache<N> &rc)
+{
+   static_assert(Fixed<N>::P.threads == kFixedThreads, "groups of several transforms: 256 work items");
+   constexpr int      FPB    = Fixed<N>::P.FPB;
+   constexpr uint32_t STRIDE = fixed_stride<N>();
+   constexpr int      iters  = R2CSplitCache<N>::iters;
+   uint32_t           w      = threadIdx.x;
+   asm volatile("" : "+v"(w));   // slots recomputed per group, not hoisted into registers
+#pragma unroll
+   for (int f = 0; f < FPB; ++f) {
+      if ((uint32_t) f >= nf) break;
+      const float2 *src = smem + f * STRIDE;
+      float2       *out = a.y + (size_t) (first + (uint32_t) f) * a.outStride;
+      float2        zi[iters], zm[iters];
+#pragma unroll
+      for (int c = 0; c < iters; ++c) {
+         const uint32_t i = min(w + (uint32_t) (c * kFixedThreads), (uint32_t) N - 1u);
+         zi[c]            = src[fslot(fixed_position<N>(i))];
+         zm[c]            = src[fslot(fixed_position<N>(i == 0u ? 0u : (uint32_t) N - i))];
+      }
+#pragma unroll
+      for (int c = 0; c < iters; ++c) {
+         const uint32_t i = threadIdx.x + (uint32_t) (c * kFixedThreads);
+         if (N % kFixedThreads != 0 && i >= (uint32_t) N) break;
+         const float2 A = rc.A[c], B = rc.B[c];
+         float2       t1;
+         if (i == 0u) t1 = make_float2(zi[c].x * B.x + zi[c].y * B.y, zi[c].x * B.x - zi[c].y * B.y);
+         else t1 = make_float2(zm[c].x * B.x + zm[c].y * B.y, zm[c].x * B.y - zm[c].y * B.x);
+         const float2 y = make_float2(zi[c].x * A.x - zi[c].y * A.y + t1.x, zi[c].x * A.y + zi[c].y * A.x + t1.y);
+         out[i]         = y;
+         if (a.aux && i > 0u) out[2u * N - i] = make_float2(y.x, -y.y);
+         if (i == 0u) out[N] = make_float2(zi[c].x - zi[c].y, 0.0f);
+      }
+   }
+}
+
 // C2C / C2R store of a group whose outputs are contiguous (no output offsets,
 // stride N complex points) for lengths with at most 8 work items per
 // transform (the C2R output is the inner complex FFT's): work item w
@@ -893,9 +956,19 @@ __device__ __forceinline__ uint32_t group_item(uint32_t t)
    return t;
 }
 
+// The dedicated kernels address their channels by stride only (launch_fixed
+// refuses offset tables, which only the class kernels take): with the
+// tables cleared here the compiler drops their loads. A table lookup in the
+// address of a prefetched load or of a store was a vector-memory load whose
+// wait, the counter completing in order, also waited for every load and
+// store before it: the prefetched next group before each store phase, the
+// previous group's stores before each prefetch.
 template <int N, int KIND>
-__global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs a)
+__global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs args)
 {
+   FixedArgs a = args;
+   a.inOff     = nullptr;
+   a.outOff    = nullptr;
    constexpr FixedPlan P      = Fixed<N>::P;
    constexpr int       T      = P.T;
    constexpr int       FPB    = P.FPB;
@@ -911,6 +984,9 @@ __global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs a)
 
    if constexpr (fixed_prefetch<N, KIND>()) {
       using Q = Pass0<N>;
+      constexpr bool kR2CCache = KIND == KIND_R2C && r2c_group_store<N>();
+      R2CSplitCache<kR2CCache ? N : 256> rc;
+      if constexpr (kR2CCache) rc.init(a);
       float2   v[Q::iters][Q::R];
       uint32_t g = blockIdx.x;
       if (g < a.numGroups) pass0_load<N, KIND>(v, g * FPB, fl, min((uint32_t) FPB, a.numChannels - g * FPB), t, a);
@@ -927,7 +1003,8 @@ __global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs a)
          if (gn < a.numGroups) pass0_load<N, KIND>(v, gn * FPB, fl, min((uint32_t) FPB, a.numChannels - gn * FPB), tg, a);
          __syncthreads();
          fixed_passes_from<N, 1>(data, tg, twp);
-         group_store<N, KIND>(smem, data, first, nf, fl, tg, a, map);
+         if constexpr (kR2CCache) r2c_store_group_cached<N>(smem, first, nf, a, rc);
+         else group_store<N, KIND>(smem, data, first, nf, fl, tg, a, map);
          __syncthreads();
       }
    }
@@ -1596,6 +1673,7 @@ template <int N, int KIND>
 hipError_t launch_fixed(FixedArgs a, hipStream_t stream, uint32_t maxGrid = 0)
 {
    constexpr FixedPlan P   = Fixed<N>::P;
+   if (a.inOff || a.outOff) return hipErrorInvalidValue;   // strided channels only
    const size_t        lds = fixed_lds_bytes<N>();
    static bool         ldsSet = false;   // set once, so launches can be captured in a graph
    if (lds > 65536 && !ldsSet) {
