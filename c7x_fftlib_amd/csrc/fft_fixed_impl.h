@@ -408,6 +408,21 @@ struct StoreMap {
    }
 };
 
+// C2R inMode 0 pre-split of a whole group when a workgroup holds FPB > 1
+// transforms: work item w takes the mirror pairs (i, N - i), i = w + 256c <
+// N/2, of every transform of the group, loading the four split factors of a
+// pair once (pair 0 takes those of N/2 for the self-mirrored Z[N/2], its
+// partner being the Nyquist bin). Same formula per point as the per-transform
+// pre-split in fixed_kernel. Used where every work item has pairs (N/2 >= 256)
+// and a group holds 2-4 transforms: C2R 2048 (503 MB per call) 244.0 ->
+// 220.4 us, 960 248.6 -> 217.3 us, 1024 239.9 -> 232.4 us; at 512 (8 per
+// group) it was slower (226.7 -> 249.1 us).
+template <int N>
+constexpr bool c2r_group_presplit()
+{
+   return Fixed<N>::P.FPB > 1 && Fixed<N>::P.FPB <= 4 && N / 2 >= kFixedThreads;
+}
+
 // Writes transform `ch` from its LDS slots `src`: work item t handles outputs
 // t, t + T, ... in chunks of up to 16 (8 for R2C; all LDS reads of a chunk
 // are issued before its stores). C2C and C2R write the N complex values in natural
@@ -812,21 +827,6 @@ __device__ __forceinline__ void stage_group(const FixedArgs &a, uint32_t first, 
    }
 }
 
-// C2R inMode 0 pre-split of a whole group when a workgroup holds FPB > 1
-// transforms: work item w takes the mirror pairs (i, N - i), i = w + 256c <
-// N/2, of every transform of the group, loading the four split factors of a
-// pair once (pair 0 takes those of N/2 for the self-mirrored Z[N/2], its
-// partner being the Nyquist bin). Same formula per point as the per-transform
-// pre-split in fixed_kernel. Used where every work item has pairs (N/2 >= 256)
-// and a group holds 2-4 transforms: C2R 2048 (503 MB per call) 244.0 ->
-// 220.4 us, 960 248.6 -> 217.3 us, 1024 239.9 -> 232.4 us; at 512 (8 per
-// group) it was slower (226.7 -> 249.1 us).
-template <int N>
-constexpr bool c2r_group_presplit()
-{
-   return Fixed<N>::P.FPB > 1 && Fixed<N>::P.FPB <= 4 && N / 2 >= kFixedThreads;
-}
-
 __device__ __forceinline__ float2 c2r_presplit(float2 x, float2 xm, float2 A, float2 B)
 {
    return make_float2(x.x * A.x - x.y * A.y + (xm.x * B.x + xm.y * B.y),
@@ -887,6 +887,90 @@ __device__ __forceinline__ void c2r_presplit_group(const FixedArgs &a, uint32_t 
       }
    }
 }
+
+// Prefetching form of the inMode 0 group pre-split (lengths with
+// c2r_group_presplit and 4 transforms per group): the four split factors of
+// each of a work item's mirror pairs are loaded once per workgroup
+// (C2RSplitCache), and the next group's bins (both halves of every pair and
+// the Nyquist bin of each transform) are loaded before the current group's
+// LDS passes (C2RGroupInput), so the passes overlap the next group's HBM
+// reads. Same-box A/B, 537 MB per call: C2R 2048 (inner 1024) 109.9 ->
+// 95.9 us; at inner 2048 (2 transforms per group, 168 VGPRs) it was slower
+// (104.9 -> 117.7 us), so that length keeps the plain group pre-split.
+template <int N>
+constexpr bool c2r_prefetch()
+{
+   return c2r_group_presplit<N>() && Fixed<N>::P.FPB >= 4;
+}
+
+template <int N>
+struct C2RSplitCache {
+   static constexpr uint32_t H  = (uint32_t) N / 2u;
+   static constexpr int      GP = (int) ((H + kFixedThreads - 1) / kFixedThreads);
+   float2                    Ai[GP], Bi[GP], Aj[GP], Bj[GP];
+
+   __device__ __forceinline__ void init(const FixedArgs &a)
+   {
+#pragma unroll
+      for (int c = 0; c < GP; ++c) {
+         const uint32_t i = min(threadIdx.x + (uint32_t) (c * kFixedThreads), H - 1u);
+         const uint32_t j = i == 0u ? H : (uint32_t) N - i;
+         Ai[c]            = split_factor(a, i);
+         Bi[c]            = split_factor(a, (uint32_t) N + i);
+         Aj[c]            = split_factor(a, j);
+         Bj[c]            = split_factor(a, (uint32_t) N + j);
+      }
+   }
+};
+
+template <int N>
+struct C2RGroupInput {
+   static constexpr int      FPB = Fixed<N>::P.FPB;
+   static constexpr uint32_t H   = C2RSplitCache<N>::H;
+   static constexpr int      GP  = C2RSplitCache<N>::GP;
+   float2                    xi[FPB][GP], xm[FPB][GP], xh[FPB];
+
+   // transforms past the batch re-read its last one and are not computed
+   __device__ __forceinline__ void load(const FixedArgs &a, uint32_t first, uint32_t nf)
+   {
+#pragma unroll
+      for (int f = 0; f < FPB; ++f) {
+         const float2 *in = a.x + (size_t) (first + min((uint32_t) f, nf - 1u)) * a.inStride;
+#pragma unroll
+         for (int c = 0; c < GP; ++c) {
+            const uint32_t i = min(threadIdx.x + (uint32_t) (c * kFixedThreads), H - 1u);
+            xi[f][c]         = in[i];
+            xm[f][c]         = in[(uint32_t) N - i];   // i = 0: the Nyquist bin
+         }
+         xh[f] = in[H];
+      }
+   }
+
+   __device__ __forceinline__ void compute(const C2RSplitCache<N> &cc, uint32_t nf, InScale sc, float2 *smem) const
+   {
+      constexpr uint32_t STRIDE = fixed_stride<N>();
+#pragma unroll
+      for (int f = 0; f < FPB; ++f) {
+         if ((uint32_t) f >= nf) break;
+         float2 *dst = smem + f * STRIDE;
+#pragma unroll
+         for (int c = 0; c < GP; ++c) {
+            const uint32_t i = threadIdx.x + (uint32_t) (c * kFixedThreads);
+            if (H % kFixedThreads != 0 && i >= H) break;
+            const float2 zi = c2r_presplit(xi[f][c], xm[f][c], cc.Ai[c], cc.Bi[c]);
+            dst[fslot(i)]   = make_float2(sc.sx * zi.x, sc.sy * zi.y);
+            if (i != 0u) {
+               const float2 zj              = c2r_presplit(xm[f][c], xi[f][c], cc.Aj[c], cc.Bj[c]);
+               dst[fslot((uint32_t) N - i)] = make_float2(sc.sx * zj.x, sc.sy * zj.y);
+            }
+            else {
+               const float2 zh = c2r_presplit(xh[f], xh[f], cc.Aj[c], cc.Bj[c]);
+               dst[fslot(H)]   = make_float2(sc.sx * zh.x, sc.sy * zh.y);
+            }
+         }
+      }
+   }
+};
 
 template <int N>
 constexpr bool fixed_direct()
@@ -963,8 +1047,18 @@ __device__ __forceinline__ uint32_t group_item(uint32_t t)
 // wait, the counter completing in order, also waited for every load and
 // store before it: the prefetched next group before each store phase, the
 // previous group's stores before each prefetch.
+// C2R kernels are compiled for at least 3 waves per SIMD: with it the
+// prefetching pre-split of C2R 2048 (159 VGPRs either way) ran 95.9 instead
+// of 99.4 us per 537 MB (same-box A/B).
+template <int KIND>
+constexpr int fixed_min_waves()
+{
+   return KIND == KIND_C2R ? 3 : 1;
+}
+
 template <int N, int KIND>
-__global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs args)
+__global__ void __launch_bounds__(Fixed<N>::P.threads) __attribute__((amdgpu_waves_per_eu(fixed_min_waves<KIND>())))
+fixed_kernel(FixedArgs args)
 {
    FixedArgs a = args;
    a.inOff     = nullptr;
@@ -1018,6 +1112,29 @@ __global__ void __launch_bounds__(Fixed<N>::P.threads) fixed_kernel(FixedArgs ar
       // at a time.
       if constexpr (fixed_tw_lds<N>()) {
          for (uint32_t i = threadIdx.x; i < (uint32_t) P.twCount; i += P.threads) smem[FPB * STRIDE + i] = a.tw[i];
+      }
+      if constexpr (c2r_prefetch<N>()) {
+         if (a.aux == 0u) {
+            C2RSplitCache<N> cc;
+            cc.init(a);
+            C2RGroupInput<N> in;
+            uint32_t         g = blockIdx.x;
+            if (g < a.numGroups) in.load(a, g * FPB, min((uint32_t) FPB, a.numChannels - g * FPB));
+            for (; g < a.numGroups; g += gridDim.x) {
+               const uint32_t first = g * FPB;
+               const uint32_t nf    = min((uint32_t) FPB, a.numChannels - first);
+               uint32_t       tg    = t;   // per-group LDS slots and offsets, not hoisted into registers
+               asm volatile("" : "+v"(tg));
+               in.compute(cc, nf, sc, smem);
+               const uint32_t gn = g + gridDim.x;
+               if (gn < a.numGroups) in.load(a, gn * FPB, min((uint32_t) FPB, a.numChannels - gn * FPB));
+               __syncthreads();
+               fixed_passes_from<N, 0>(data, tg, twp);
+               group_store<N, KIND>(smem, data, first, nf, fl, tg, a, map);
+               __syncthreads();
+            }
+            return;
+         }
       }
       constexpr int iters = N / T;
       constexpr int CH    = largest_divisor_upto(iters, 8);
