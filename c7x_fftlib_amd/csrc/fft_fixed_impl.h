@@ -862,6 +862,10 @@ __device__ __forceinline__ void c2r_presplit_group(const FixedArgs &a, uint32_t 
          const float2 *in  = a.x + in_base(a, first + (uint32_t) f);
          float2       *dst = smem + f * STRIDE;
          float2        xi[GCH], xm[GCH];
+         // bin N/2, the partner of Z[N/2] that work item 0 forms, loaded with
+         // the others by every work item (one address per wavefront): loaded
+         // where it is used, its round trip held up the whole group
+         const float2 xh = in[H];
 #pragma unroll
          for (int c = 0; c < GCH; ++c) {
             const uint32_t i = min(threadIdx.x + (uint32_t) ((c0 + c) * kFixedThreads), H - 1u);
@@ -879,7 +883,6 @@ __device__ __forceinline__ void c2r_presplit_group(const FixedArgs &a, uint32_t 
                dst[fslot((uint32_t) N - i)] = make_float2(sc.sx * zj.x, sc.sy * zj.y);
             }
             else {
-               const float2 xh = in[H];
                const float2 zh = c2r_presplit(xh, xh, Aj[c], Bj[c]);
                dst[fslot(H)]   = make_float2(sc.sx * zh.x, sc.sy * zh.y);
             }
@@ -1153,6 +1156,7 @@ fixed_kernel(FixedArgs args)
          }
          else if (fl < nf && a.aux == 0u) {
             const float2 *in = a.x + in_base(a, first + fl);
+            const float2  xh = in[N / 2];   // bin N/2 for Z[N/2], loaded up front by every work item
 #pragma unroll 1
             for (int c0 = 0; c0 < HALF; c0 += PCH) {
                float2 xi[PCH], xm[PCH];
@@ -1177,7 +1181,6 @@ fixed_kernel(FixedArgs args)
                   }
                   else {
                      constexpr uint32_t H  = (uint32_t) N / 2u;
-                     const float2       xh = in[H];
                      const float2       zh = presplit(xh, xh, split_factor(a, H), split_factor(a, (uint32_t) N + H));
                      data[fslot(H)]        = make_float2(sc.sx * zh.x, sc.sy * zh.y);
                   }
