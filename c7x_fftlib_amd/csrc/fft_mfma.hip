@@ -2,16 +2,15 @@
 // the matrix cores for the lengths N = R1 * 16 * R3 (R1 <= 16, R3 in
 // {1, 2, 4}) listed in FFTLIB_MFMA_LENGTHS below: 21 lengths from 32 to 1024.
 //
-// With C = 16 R3 columns, n = C a + b (a < R1, b < C), b = R3 b1 + b2
-// (b1 < 16, b2 < R3) and k = c + R1 d1 + 16 R1 d2 (c < R1, d1 < 16, d2 < R3):
+// With C = 16 R3 columns, n = C a + b (a < R1, b < C), b = 16 b2 + b1
+// (b1 < 16, b2 < R3) and k = c + R1 d2 + R1 R3 d1 (c < R1, d2 < R3, d1 < 16):
 //
 //   Y1[c][b]     = sum_a W_R1^(ac) x[C a + b]              radix-R1 stage (GEMM 1)
-//   Y2[c][b]     = W_N^(bc) Y1[c][b]                        twiddle
-//   V[c][b2][d1] = sum_b1 W_16^(b1 d1) Y2[c][R3 b1 + b2]    radix-16 stage (GEMM 2)
-//   X[k]         = sum_b2 W_R3^(b2 d2) W_C^(b2 d1) V[c][b2][d1]
-//                                                          radix-R3 stage (fp32 VALU)
+//   U[c][d2][b1] = sum_b2 W_R3^(b2 d2) W_N^(bc) Y1[c][b]    twiddle, radix-R3 stage (fp32 VALU)
+//   X[k]         = sum_b1 W_16^(b1 d1) W_C^(b1 d2) U[c][d2][b1]
+//                                                          twiddle, radix-16 stage (GEMM 2)
 //
-// (for N = 960: the radix-15 and radix-64 = 16 x 4 DFT stages). The outputs
+// (for N = 960: the radix-15 and radix-64 = 4 x 16 DFT stages). The outputs
 // go through a per-wavefront LDS staging area so that every store
 // instruction writes 64 consecutive points. Both DFT
 // stages run in real form on v_mfma_f32_16x16x32_{f16,bf16} with FP32
@@ -19,17 +18,17 @@
 // A operand (rows = the 16 columns b of a tile, K = the real and imaginary
 // parts of x over a) and the DFT matrix the B operand; in GEMM 2 the DFT
 // matrix is the A operand and the data the B operand (K = b1, columns = c).
-// The K order of GEMM 2 is chosen so that the accumulator layout of GEMM 1
-// (work item (q, r) = (lane / 16, lane % 16) holds rows b = 16 mt + 4q + i
-// of column c = r) is exactly the B-operand layout of GEMM 2: for every b2
-// work item q holds four values of b1, and K index 8q + 2e + p is taken to be
-// part p of the e-th of them. The data therefore never leaves the registers
-// between the stages, and the radix-R3 stage finds all b2 of its outputs in
-// one work item. The input is read straight into the GEMM-1 operand layout
-// (K index 8q + 2jj + p is part p of x[C (4q + jj) + b]: four 8-byte loads
-// per tile, whole 128-byte lines per wavefront load). Besides the output
-// staging, LDS holds the DFT-matrix fragments and the W_N table, shared by
-// the workgroup.
+// Tile mt of GEMM 1 holds the rows b = 16 mt + 4q + i (work item (q, r) =
+// (lane / 16, lane % 16), i < 4) of column c = r, i.e. b2 = mt and b1 = 4q +
+// i: the R3 inputs of every radix-R3 butterfly are in one work item, and after
+// that stage tile d2 of the accumulators is exactly the B operand of GEMM 2's
+// N-tile d2 with K index 8q + 2i + p = part p of b1 = 4q + i. The data
+// therefore never leaves the registers between the stages, and each GEMM-2
+// tile frees the accumulators it consumed. The input is read straight into
+// the GEMM-1 operand layout (K index 8q + 2jj + p is part p of
+// x[C (4q + jj) + b]: four 8-byte loads per tile, whole 128-byte lines per
+// wavefront load). Besides the output staging, LDS holds the DFT-matrix
+// fragments and the twiddles, shared by the workgroup.
 //
 // The data operands are split into sums of 16-bit parts so that the products
 // recover FP32-level accuracy:
@@ -189,7 +188,7 @@ struct MfmaArgs {
    const float2   *x;
    float2         *y;
    const uint16_t *frag;      // GEMM-1 B fragments of every part, then GEMM-2 A fragments of every part
-   const float2   *twN;       // W_N^m, m < N
+   const float2   *twN;       // twiddle rows: W_N^(r b) at r C + b (r < 16, b < C), then W_C^(b1 d2) at 16 C + 16 d2 + b1
    uint32_t        numChannels;
    int             mode;      // C2CMode
 };
@@ -200,6 +199,17 @@ __device__ __forceinline__ float wave_max(float v)
 #pragma unroll
    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
    return v;
+}
+
+// Orders this wavefront's LDS stores before its other lanes' later LDS
+// accesses: the LDS completes one wavefront's operations in order, and the
+// fences and the wave barrier keep the compiler from moving or forwarding
+// loads and stores across this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 w)
@@ -214,14 +224,24 @@ constexpr int frag_halves()
    return 2 * SplitTraits<S>::parts * 64 * 8;
 }
 
+// Twiddle entries in LDS: the rows W_N^(r b) (r < 16, b < C = 16 R3), so that
+// each work item reads its 4 R3 twiddles at one base plus immediate offsets,
+// then W_C^(b1 d2) at 16 d2 + b1.
+template <int R3>
+constexpr int mfma_tw_entries()
+{
+   return 16 * 16 * R3 + 16 * R3;
+}
+
 // Outputs are staged in LDS per wavefront and stored 64 consecutive points
 // per instruction: stored straight from registers (runs of R1 points, not
 // line aligned for R1 = 15) N = 960 took 360-405 us instead of 191-198 us.
+
 template <int S, int R1, int R3>
 constexpr size_t mfma_lds_bytes()
 {
    return (size_t) 2 * frag_halves<S>() * sizeof(uint16_t) +
-          (size_t) R1 * 16 * R3 * sizeof(float2) * (1 + kWaves);
+          ((size_t) mfma_tw_entries<R3>() + (size_t) R1 * 16 * R3 * kWaves) * sizeof(float2);
 }
 
 // Wavefronts per SIMD the register allocation targets: 3 for the shapes
@@ -239,20 +259,22 @@ c2c_mfma_kernel(MfmaArgs a)
 {
    constexpr int P = SplitTraits<S>::parts;
    constexpr int C = 16 * R3;
-   constexpr int N = R1 * C;
-   constexpr int E = 4 / R3;   // GEMM-2 K elements e of one tile mt: (mt, i) = (e / E, R3 (e % E) + b2)
+   constexpr int N   = R1 * C;
+   constexpr int kTw = mfma_tw_entries<R3>();
    extern __shared__ __align__(16) unsigned char ldsm[];
    u16x8  *f1 = (u16x8 *) ldsm;   // [t][part][lane]
    u16x8  *f2 = f1 + 2 * P * 64;
    float2 *tw = (float2 *) (f2 + 2 * P * 64);
    for (uint32_t i = threadIdx.x; i < (uint32_t) (4 * P * 64); i += kThreadsM) f1[i] = ((const u16x8 *) a.frag)[i];
-   for (uint32_t i = threadIdx.x; i < (uint32_t) N; i += kThreadsM) tw[i] = a.twN[i];
+   for (uint32_t i = threadIdx.x; i < (uint32_t) kTw; i += kThreadsM) tw[i] = a.twN[i];
    __syncthreads();
 
    const uint32_t lane = threadIdx.x % 64;
    const uint32_t wave = threadIdx.x / 64;
    const uint32_t q    = lane / 16, r = lane % 16;
    const bool     live = r < (uint32_t) R1;   // column c = r exists (the others are padding)
+   const float2  *twr  = tw + (live ? r * C : 0u) + 4u * q;   // W_N^(r (16 mt + 4q + i)) at 16 mt + i
+   const float2  *twc  = tw + 16 * C + 4u * q;                 // W_C^((4q + i) d2) at 16 d2 + i
 
    const float sxIn  = a.mode == C2C_INVERSE_PRESCALE ? 1.0f / (float) N : 1.0f;
    const float syIn  = a.mode == C2C_FORWARD ? sxIn : -sxIn;
@@ -303,11 +325,11 @@ c2c_mfma_kernel(MfmaArgs a)
       // ---- GEMM 1 (transposed): rows b = 16 mt + (0..15), K = parts of x over a ---
       f32x4 acc1[R3][2];
       {
-         u16x8 F[2][P];
+         u16x8 F1[2][P];
 #pragma unroll
          for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int p = 0; p < P; ++p) F[t][p] = f1[(t * P + p) * 64 + lane];
+            for (int p = 0; p < P; ++p) F1[t][p] = f1[(t * P + p) * 64 + lane];
 #pragma unroll
          for (int mt = 0; mt < R3; ++mt) {
             float vv[8];
@@ -317,51 +339,20 @@ c2c_mfma_kernel(MfmaArgs a)
             split8<S>(vv, A);
             if (more) load_tile(v[mt], a.x + (size_t) fn * N, mt);   // tile mt of the next transform
 #pragma unroll
-            for (int t = 0; t < 2; ++t) acc1[mt][t] = split_mma<S>(A, F[t], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            for (int t = 0; t < 2; ++t) acc1[mt][t] = split_mma<S>(A, F1[t], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
             __builtin_amdgcn_sched_barrier(0);   // one tile at a time: bounds the live registers
          }
       }
-      // ---- twiddle W_N^(b c) and GEMM 2 (transposed) --------------------------------
-      // Element (mt, i) of GEMM 1 is row b = 16 mt + 4q + i of column c = r.
-      // GEMM 2 N-tile u = b2 takes K index 8q + 2e + p = part p of element
-      // (mt, i) = (e / E, R3 (e % E) + u), twiddled as it is consumed.
-      f32x4 acc2[R3][2];
-      {
-         u16x8 F[2][P];
-#pragma unroll
-         for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int p = 0; p < P; ++p) F[t][p] = f2[(t * P + p) * 64 + lane];
-#pragma unroll
-         for (int u = 0; u < R3; ++u) {
-            float vv[8];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-               const int      mt = e / E, i = R3 * (e % E) + u;
-               const uint32_t b  = 16u * mt + 4u * q + (uint32_t) i;
-               const float2   z  = cmul(make_float2(acc1[mt][0][i], acc1[mt][1][i]), tw[live ? b * r : 0u]);
-               vv[2 * e]         = z.x;
-               vv[2 * e + 1]     = z.y;
-            }
-            u16x8 B[P];
-            split8<S>(vv, B);
-#pragma unroll
-            for (int t = 0; t < 2; ++t) acc2[u][t] = split_mma<S>(F[t], B, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-            __builtin_amdgcn_sched_barrier(0);   // one tile at a time: bounds the live registers
-         }
-      }
-      // ---- radix-R3 stage with W_C^(b2 d1) = W_N^(R1 b2 d1); store ----------------
-      float2 *yo = a.y + (size_t) f * N;
-      float2 *zs = tw + N + wave * N;   // this wavefront's output staging
+      // ---- twiddle W_N^(b c), radix-R3 stage over b2, twiddle W_C^(b1 d2) ----------
+      // Element i of tile mt is row b = 16 mt + 4q + i of column c = r, so
+      // b2 = mt and b1 = 4q + i: the R3 inputs of each radix-R3 butterfly sit in
+      // one work item, and the stage runs in place in acc1.
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-         const uint32_t d1 = 4u * q + (uint32_t) i;
-         float2         t[R3];
+         float2 t[R3];
 #pragma unroll
-         for (int u = 0; u < R3; ++u) {
-            t[u] = make_float2(acc2[u][0][i], acc2[u][1][i]);
-            if (u > 0) t[u] = cmul(t[u], tw[(uint32_t) (R1 * u) * d1]);
-         }
+         for (int mt = 0; mt < R3; ++mt)
+            t[mt] = cmul(make_float2(acc1[mt][0][i], acc1[mt][1][i]), twr[16 * mt + i]);
          float2 z[R3];
          if constexpr (R3 == 1) {
             z[0] = t[0];
@@ -380,18 +371,52 @@ c2c_mfma_kernel(MfmaArgs a)
             z[2]             = make_float2(s02.x - s13.x, s02.y - s13.y);
             z[3]             = make_float2(d02.x - d13.y, d02.y + d13.x);   // d02 + i d13
          }
-         if (live) {
 #pragma unroll
-            for (int d2 = 0; d2 < R3; ++d2) {
-               const float zr = unscale * z[d2].x, zi = unscale * z[d2].y;
-               const float2 o  = pdiv ? make_float2(zr / (float) N, -(zi / (float) N)) : make_float2(zr, syOut * zi);
-               zs[r + R1 * d1 + 16 * R1 * d2] = o;
-            }
+         for (int d2 = 0; d2 < R3; ++d2) {
+            if (d2 > 0) z[d2] = cmul(z[d2], twc[16 * d2 + i]);
+            acc1[d2][0][i] = z[d2].x;
+            acc1[d2][1][i] = z[d2].y;
          }
       }
+      __builtin_amdgcn_sched_barrier(0);
+      // ---- GEMM 2 (transposed), N-tile d2: K index 8q + 2e + p = part p of element
+      // e (b1 = 4q + e) of tile d2; tile d2 of acc1 is dead once split ---------------
+      f32x4 acc2[R3][2];
+      u16x8 F2[2][P];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+         for (int p = 0; p < P; ++p) F2[t][p] = f2[(t * P + p) * 64 + lane];
+#pragma unroll
+      for (int d2 = 0; d2 < R3; ++d2) {
+         float vv[8];
+#pragma unroll
+         for (int e = 0; e < 4; ++e) vv[2 * e] = acc1[d2][0][e], vv[2 * e + 1] = acc1[d2][1][e];
+         u16x8 B[P];
+         split8<S>(vv, B);
+#pragma unroll
+         for (int t = 0; t < 2; ++t) acc2[d2][t] = split_mma<S>(F2[t], B, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+         __builtin_amdgcn_sched_barrier(0);   // one tile at a time: bounds the live registers
+      }
+      // ---- store X[r + R1 (d2 + R3 d1)] (d1 = 4q + i: row of the GEMM-2 output) ------
+      // through this wavefront's LDS staging; the inverse modes conjugate, POSTDIV
+      // divides by N as a product with the fp32 reciprocal.
+      float2 *yo = a.y + (size_t) f * N;
+      float2 *zs = tw + kTw + wave * N;
+      const float ox = pdiv ? unscale * (1.0f / (float) N) : unscale;
+      const float oy = pdiv ? -ox : syOut * unscale;
+      if (live) {
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int d2 = 0; d2 < R3; ++d2)
+               zs[r + R1 * (d2 + R3 * (4 * q + i))] = make_float2(ox * acc2[d2][0][i], oy * acc2[d2][1][i]);
+      }
+      wave_lds_sync();
 #pragma unroll
       for (int j = 0; j < (N + 63) / 64; ++j)
          if (N % 64 == 0 || lane + 64u * j < (uint32_t) N) engine::store_stream(yo + lane + 64 * j, zs[lane + 64 * j]);
+      wave_lds_sync();   // the next transform's staging stores come after these loads
    }
 }
 
@@ -462,13 +487,11 @@ double real_form(int t, int p, int jk, int m)
 //   GEMM-1 B operand: B[k = 8q + j][n = 16t + r] = coefficient of part p of
 //     x[a = 4q + e] in part t of Y1[c = r] (0 when a or c is at least R1);
 //   GEMM-2 A operand: A[m = 16t + r][k = 8q + j] = coefficient of part p of
-//     Y2[b1] in part t of V[d1 = r], b1 = 16 mt / R3 + 4q / R3 + e % E with
-//     mt = e / E, E = 4 / R3 (the b1 of element e of work item q).
+//     Z[b1 = 4q + e] in part t of V[d1 = r].
 // Layout: [table][t][part][lane][8 halves].
 std::vector<uint16_t> build_frags(int S, int R1, int R3)
 {
    const int             P = S == SPLIT_BF16X3 ? 3 : 2;
-   const int             E = 4 / R3;
    std::vector<uint16_t> h((size_t) 2 * 2 * P * 64 * 8);
    uint16_t              parts[3];
    for (int tab = 0; tab < 2; ++tab)
@@ -482,8 +505,7 @@ std::vector<uint16_t> build_frags(int S, int R1, int R3)
                   if (aa < R1 && r < R1) v = real_form(t, p, aa * r, R1);
                }
                else {
-                  const int b1 = 16 * (e / E) / R3 + 4 * q / R3 + e % E;
-                  v            = real_form(t, p, b1 * r, 16);
+                  v = real_form(t, p, (4 * q + e) * r, 16);
                }
                host_split(S, v, parts);
                for (int pp = 0; pp < P; ++pp) h[((((size_t) tab * 2 + t) * P + pp) * 64 + l) * 8 + j] = parts[pp];
@@ -515,20 +537,25 @@ hipError_t get_tables(int S, uint32_t n, int R1, int R3, const uint16_t **frag, 
    auto it = g_tables.find(TableKey{dev, S, n});
    if (it == g_tables.end()) {
       const std::vector<uint16_t> h = build_frags(S, R1, R3);
-      std::vector<float2>         w(n);
+      const uint32_t              C = 16u * (uint32_t) R3, entries = 16u * C + 16u * (uint32_t) R3;
+      std::vector<float2>         w(entries);
       const double                pi = 3.14159265358979323846;
-      for (uint32_t m = 0; m < n; ++m) {
-         const double ang = -2.0 * pi * (double) m / (double) n;
-         w[m]             = make_float2((float) cos(ang), (float) sin(ang));
-      }
+      auto                        wn = [&](uint32_t m) {
+         const double ang = -2.0 * pi * (double) (m % n) / (double) n;
+         return make_float2((float) cos(ang), (float) sin(ang));
+      };
+      for (uint32_t r = 0; r < 16; ++r)
+         for (uint32_t b = 0; b < C; ++b) w[r * C + b] = wn(r * b);
+      for (uint32_t d2 = 0; d2 < (uint32_t) R3; ++d2)
+         for (uint32_t b1 = 0; b1 < 16; ++b1) w[16 * C + 16 * d2 + b1] = wn((uint32_t) R1 * b1 * d2);
       DeviceTables t;
       if ((e = hipMalloc((void **) &t.frag, h.size() * sizeof(uint16_t))) != hipSuccess) return e;
-      if ((e = hipMalloc((void **) &t.twN, n * sizeof(float2))) != hipSuccess) {
+      if ((e = hipMalloc((void **) &t.twN, entries * sizeof(float2))) != hipSuccess) {
          (void) hipFree(t.frag);
          return e;
       }
       if ((e = hipMemcpy(t.frag, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) != hipSuccess ||
-          (e = hipMemcpy(t.twN, w.data(), n * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
+          (e = hipMemcpy(t.twN, w.data(), entries * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
          (void) hipFree(t.frag);
          (void) hipFree(t.twN);
          return e;
