@@ -42,6 +42,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+RAGGED_FORMS = {"graph": 0, "persistent": 1}   # FFTLIB_RAGGED_GRAPH, FFTLIB_RAGGED_PERSISTENT
+RAGGED_KERNELS = {
+    "persistent": "ragged_class_kernel (one persistent launch per direction: lengths <= 4096 whole, longer "
+                  "lengths as column and row groups that wait inside the launch)",
+    "graph": "ragged graph (class kernels: lengths <= 16384 in one pass, longer lengths in a column and a row pass)",
+}
 METRIC = "Batched C2C FFT GFLOP/s & %HBM peak; split-BF16 MFMA vs FP32 at ≤2e-6 err"
 HBM_PEAK_GBS = 8000.0
 
@@ -54,6 +60,8 @@ def parse():
     ap.add_argument("--workload", default="c2c_960", choices=["c2c_960", "i16_256", "r2c_4096", "ragged_10000", "copy_503m",
                              "c2c_960_host"])
     ap.add_argument("--no-tune", action="store_true", help="ragged_10000: skip the autotuning pass")
+    ap.add_argument("--ragged-form", default="auto", choices=["auto", "persistent", "graph"],
+                    help="ragged_10000: plan form (auto: the autotuner's choice; without tuning the graph form)")
     ap.add_argument("--variant", default="fp32", choices=["fp32", "bf16x3", "fp16x2", "bf16x2"],
                     help="C2C arithmetic: fp32 mixed radix, or MFMA DFT-as-GEMM with split bf16/fp16 operands")
     ap.add_argument("--generic", action="store_true",
@@ -284,6 +292,8 @@ class Workload:
         st = lib.FFTLIB_GPU_raggedCreate(ctypes.byref(self.h), arr, len(lengths))
         if st != 0:
             raise RuntimeError(f"raggedCreate status {st}: {F.last_error()}")
+        if self.args.ragged_form != "auto":
+            assert lib.FFTLIB_GPU_raggedSetForm(self.h, RAGGED_FORMS[self.args.ragged_form]) == 0
         r = philox(10001)
         host = np.empty(total, np.complex64)
         chunk = 1 << 24
@@ -299,6 +309,7 @@ class Workload:
             st = lib.FFTLIB_GPU_raggedTune(self.h, self.x.data_ptr(), self.y.data_ptr())
             if st != 0:
                 raise RuntimeError(f"raggedTune status {st}: {F.last_error()}")
+        self.form = {v: k for k, v in RAGGED_FORMS.items()}[lib.FFTLIB_GPU_raggedGetForm(self.h)]
         lg = np.log2(lengths.astype(np.float64))
         self.flops = float(2 * (5.0 * lengths * lg).sum())
         self.bytes = float(2 * 16.0 * total)
@@ -307,9 +318,9 @@ class Workload:
         self.config = {"workload": "ragged batch of 10000 C2C FFTs, lengths log-uniform on [64, 65536] snapped to "
                                    "2/3/5-smooth (seed 10000), forward then inverse in one graph launch",
                        "transforms": int(len(lengths)), "distinct_lengths": int(len(np.unique(lengths))),
-                       "points": total, "tuned": not self.args.no_tune,
+                       "points": total, "tuned": not self.args.no_tune, "form": self.form,
                        "input": "complex64 uniform[-1,1) Philox seed 10001"}
-        self.kernel_name = "ragged graph (class kernels: lengths <= 16384 in one pass, longer lengths in a column and a row pass)"
+        self.kernel_name = RAGGED_KERNELS[self.form]
         xp, yp, zp, h = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(), self.h
         fn = lib.FFTLIB_GPU_raggedExec
 
@@ -318,6 +329,29 @@ class Workload:
             if s != 0:
                 raise RuntimeError(f"raggedExec status {s}: {F.last_error()}")
         self.step = step
+
+    def extra_ragged_10000(self, steps):
+        """Both plan forms timed on the same buffers after the headline steps
+        (ms per forward + inverse step, HIP events, best of 3 x `steps`)."""
+        lib, torch = self.lib, self.torch
+        stream = torch.cuda.current_stream()
+        forms = {}
+        for name, code in RAGGED_FORMS.items():
+            assert lib.FFTLIB_GPU_raggedSetForm(self.h, code) == 0
+            for _ in range(3):
+                self.step()
+            best = float("inf")
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(steps):
+                    self.step()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1) / steps)
+            forms[name] = {"ms_per_step": round(best, 5), "kernel": RAGGED_KERNELS[name]}
+        assert lib.FFTLIB_GPU_raggedSetForm(self.h, RAGGED_FORMS[self.form]) == 0
+        return {"forms": forms}
 
     def check_ragged_10000(self):
         """Forward output of 64 transforms against the FP64 FFT; round-trip error
@@ -598,8 +632,13 @@ def time_steps(wl, args, stream, min_warmup_s):
 
 
 # Error bars of the C2C variants against the FP64 DFT: the split-precision
-# goal is 2e-6; BF16x2 (three bf16 products) is reported with its own bar.
-VARIANT_BARS = {"fp32": 1e-6, "bf16x3": 2e-6, "fp16x2": 2e-6, "bf16x2": 1e-5}
+# goal is 2e-6 for every variant. BF16x2 (three bf16 products, about 4.6e-6)
+# does not meet it: it is timed and reported with within_bar false, and only
+# the variants built to meet the bar (BF16x3, FP16x2) gate the run; its own
+# check bound (--variant bf16x2) is VARIANT_CHECK.
+VARIANT_BARS = {"fp32": 1e-6, "bf16x3": 2e-6, "fp16x2": 2e-6, "bf16x2": 2e-6}
+VARIANT_CHECK = {"fp32": 1e-6, "bf16x3": 2e-6, "fp16x2": 2e-6, "bf16x2": 1e-5}
+VARIANT_GATED = ("bf16x3", "fp16x2")
 VARIANT_CODES = {"fp32": 0, "bf16x3": 1, "fp16x2": 2, "bf16x2": 3}
 
 
@@ -607,7 +646,7 @@ def check_passes(args, chk):
     if args.workload in ("c2c_960", "c2c_960_host"):
         if args.variant == "fp32":
             return chk["rel_l2_vs_oracle"] <= 1e-6 and chk["rel_l2_vs_fp64"] <= 1e-6
-        return chk["rel_l2_vs_fp64"] <= VARIANT_BARS[args.variant]
+        return chk["rel_l2_vs_fp64"] <= VARIANT_CHECK[args.variant]
     if args.workload == "ragged_10000":
         return chk["fwd_rel_l2_vs_fp64"] <= 1e-6 and chk["roundtrip_rel_l2"] <= 1e-6
     if args.workload == "r2c_4096":
@@ -760,7 +799,7 @@ def main():
     out["check"]["passed"] = ok
     if args.workload == "c2c_960" and args.variant == "fp32" and not args.generic and not args.no_variants:
         out["variants"] = time_variants(wl, args, lib, F, torch, stream, ms)
-        ok = ok and all(v["within_bar"] for v in out["variants"].values() if v["bar"] <= 2e-6)
+        ok = ok and all(out["variants"][k]["within_bar"] for k in VARIANT_GATED)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = getattr(wl, "cpu_" + args.workload)(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -40,6 +40,9 @@ FFTLIB_FUNCTION_NATC = 0
 FFTLIB_FUNCTION_OPTIMIZED = 1
 FFTLIB_FUNCTION_GPU = 2
 
+FFTLIB_RAGGED_GRAPH = 0
+FFTLIB_RAGGED_PERSISTENT = 1
+
 SE_PARAM_SIZE = 64
 PBLOCK_SIZE = {
     "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc": 31 * SE_PARAM_SIZE + 18 * 4,
@@ -122,6 +125,8 @@ _EXTRA = {
     "FFTLIB_GPU_raggedTune": (_S, [_P, _P, _P]),
     "FFTLIB_GPU_raggedDestroy": (_S, [_P]),
     "FFTLIB_GPU_raggedPoints": (ctypes.c_uint64, [_P]),
+    "FFTLIB_GPU_raggedSetForm": (_S, [_P, ctypes.c_int32]),
+    "FFTLIB_GPU_raggedGetForm": (ctypes.c_int32, [_P]),
     "FFTLIB_GPU_tuneSave": (_S, [ctypes.c_char_p]),
     "FFTLIB_GPU_tuneLoad": (_S, [ctypes.c_char_p]),
     "FFTLIB_GPU_synchronize": (_S, []),

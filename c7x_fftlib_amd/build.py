@@ -24,6 +24,7 @@ SOURCES = [
     "fft_fixed_class1.hip",
     "fft_fixed_class2.hip",
     "fft_fixed_class3.hip",
+    "fft_ragged_class.hip",
     "fft_large.hip",
     "fft_mfma.hip",
     "fft_mma.hip",

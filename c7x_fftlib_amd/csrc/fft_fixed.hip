@@ -232,9 +232,11 @@ std::map<TwoPassKey, TwoPassPlan>   g_twoPass;   // device tables live until the
 // + inverse): 18432 = 144 x 128 runs at 1.44 TB/s, 18750 = 150 x 125 at
 // 1.16. The split maximises: both factors <= 256 (8), m2 a multiple of 16
 // (4) or 8 (2), n1 a multiple of 16 (2) or 8 (1); then the most balanced.
-bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2)
+bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2) { return two_pass_split_above(n, 16384u, n1, m2); }
+
+bool two_pass_split_above(uint32_t n, uint32_t above, uint32_t *n1, uint32_t *m2)
 {
-   if (n <= 16384u || n > kTwoPassMaxN || !smooth235(n)) return false;
+   if (n <= above || n < 256u || n > kTwoPassMaxN || !smooth235(n)) return false;
    int      a = 0, b = 0, c = 0;
    uint32_t t = n;
    while (t % 2u == 0u) t /= 2u, ++a;
@@ -276,7 +278,7 @@ size_t two_pass_scratch_points(uint32_t n, uint32_t count)
 hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan)
 {
    uint32_t n1 = 0, m2 = 0;
-   if (!two_pass_split(n, &n1, &m2)) return hipErrorInvalidValue;
+   if (!two_pass_split_above(n, kRaggedClassMaxN, &n1, &m2)) return hipErrorInvalidValue;
    int        dev = 0;
    hipError_t e   = hipGetDevice(&dev);
    if (e != hipSuccess) return e;

@@ -30,6 +30,14 @@ hipError_t fixed_class_launch_tu1(int cls, const fixedk::ClassArgs &c, uint32_t 
 hipError_t fixed_class_launch_tu2(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
 hipError_t fixed_class_launch_tu3(int cls, const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
 
+// Persistent ragged launch (fft_ragged_class.hip): every item of c (tasks
+// of the lengths up to kRaggedClassMaxN, with their wait / signal counters)
+// in one launch; grid 0 = the resident workgroups.
+constexpr uint32_t kRaggedClassMaxN = 4096;
+hipError_t ragged_class_launch(const fixedk::ClassArgs &c, uint32_t grid, hipStream_t stream);
+// Workgroups of the persistent ragged kernel resident at once on the device.
+uint32_t ragged_class_resident();
+
 // Workgroup group size (transforms per workgroup) of length n's class kernel.
 uint32_t class_group_size(uint32_t n);
 
@@ -50,13 +58,16 @@ struct TwoPassPlan {
 
 // Split of n (false when n <= 16384, n > 2^22 or not 2/3/5-smooth).
 bool two_pass_split(uint32_t n, uint32_t *n1, uint32_t *m2);
+// The same choice for any n above `above` (at least 256): the persistent
+// ragged launch splits every length above kRaggedClassMaxN.
+bool two_pass_split_above(uint32_t n, uint32_t above, uint32_t *n1, uint32_t *m2);
 
 // Complex points of the intermediate copy for `count` transforms of n points
 // (0 when n has no two-pass split).
 size_t two_pass_scratch_points(uint32_t n, uint32_t count);
 
-// Plan of n on the current device; its tables are built on first use and kept
-// until the process ends.
+// Plan of n (> kRaggedClassMaxN) on the current device; its tables are built
+// on first use and kept until the process ends.
 hipError_t two_pass_plan(uint32_t n, TwoPassPlan *plan);
 
 // Class-kernel task of one pass over `count` transforms: layout

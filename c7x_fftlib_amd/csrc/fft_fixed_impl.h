@@ -1269,6 +1269,15 @@ struct ClassTask {
    uint32_t        scrStride;    // two-pass: see FixedArgs
 };
 
+// Work unit of the persistent ragged launch: groups [g0, g1) of task, then,
+// if task2 != kNoTask, groups [h0, h1) of task2 after a workgroup barrier
+// (a long transform's column pass, then its row pass, in one workgroup).
+constexpr uint32_t kNoTask = 0xFFFFFFFFu;
+struct RaggedUnit {
+   uint32_t task, g0, g1;
+   uint32_t task2, h0, h1;
+};
+
 struct ClassArgs {
    const float2    *x;
    float2          *y;
@@ -1277,6 +1286,8 @@ struct ClassArgs {
    uint32_t         numItems;
    const ClassTask *tasks;      // null: one task, `single`, with item = group index
    const uint32_t  *items;      // (task << 20) | group
+   const RaggedUnit *units;     // ragged_class_kernel: the units, in queue order (numItems of them)
+   uint32_t        *counters;   // ragged_class_kernel: [0] queue head, [1] workgroups done
    ClassTask        single;
 };
 
@@ -1734,6 +1745,124 @@ hipError_t launch_class(const ClassArgs &c, uint32_t grid, hipStream_t stream)
    uint32_t target = (uint32_t) std::min<uint64_t>(c.numItems, 8ull * class_resident_groups<Ls...>());
    if (grid != 0) target = min(c.numItems, grid);
    hipLaunchKernelGGL((fixed_class_kernel<Ls...>), dim3(target), dim3(class_threads<Ls...>()), lds, stream, c);
+   return hipGetLastError();
+}
+
+// ---- persistent ragged launch ------------------------------------------------
+// One launch runs every transform of a ragged direction: the lengths of all
+// the kernel's classes (one workgroup size, LDS of the largest) and a work
+// queue of units instead of a fixed item-to-workgroup deal. A unit is one
+// group of a plain transform task, or a whole long transform: all groups of
+// its column pass, a workgroup barrier, all groups of its row pass. The row
+// pass reads only what the same workgroup's column pass wrote, so the
+// dependency between the passes is inside the workgroup: the barrier orders
+// the column stores before the row loads (one CU, one L1), with no
+// cross-workgroup waits, counters or device-scope fences.
+// counters: [kRqQueue] next unit, [kRqExit] workgroups done; the last
+// workgroup out clears both for the next launch of the same plan (a graph
+// replays it with no memset node).
+enum : uint32_t { kRqQueue = 0u, kRqExit = 1u, kRqCounters = 2u };
+
+// The column-pass twiddle tables of the long transforms held in LDS (staged
+// once per task as in the octave kernels), even though the extra 4 KB leaves
+// room for one workgroup fewer per CU than the class's data alone.
+#ifndef FFTLIB_RAGGED_COLTW
+#define FFTLIB_RAGGED_COLTW 1
+#endif
+constexpr bool ragged_coltw_lds() { return FFTLIB_RAGGED_COLTW != 0; }
+
+template <int... Ls>
+__global__ void __launch_bounds__(class_threads<Ls...>()) __attribute__((amdgpu_waves_per_eu(class_waves<Ls...>())))
+ragged_class_kernel(ClassArgs c)
+{
+   extern __shared__ float2 smem_class[];
+   constexpr uint32_t twSlot   = class_data_slots<Ls...>();
+   constexpr uint32_t heldSlot = twSlot + class_tw_slots<Ls...>();
+   constexpr uint32_t colSlot  = ragged_coltw_lds() ? heldSlot + 1u : 0u;
+   // queue entries (two slots) and the last-workgroup flag after the tables
+   uint32_t *sItem = (uint32_t *) (smem_class + heldSlot + 1u + (colSlot != 0u ? kColTwSlots + 1u : 0u));
+   uint32_t *sLast = sItem + 2;
+   constexpr int      kBudget  = pass0_register_points(class_waves<Ls...>());
+   uint32_t          *ctr      = c.counters;
+   if (threadIdx.x == 0) {
+      *(const float2 **) (smem_class + heldSlot) = nullptr;
+      if constexpr (colSlot != 0u) *(const float2 **) (smem_class + colSlot + kColTwSlots) = nullptr;
+   }
+   const uint64_t args = (uint64_t) __builtin_amdgcn_kernarg_segment_ptr();
+   auto           run  = [&](uint32_t ti, uint32_t g) __attribute__((always_inline)) {
+      const uint64_t task = (uint64_t) (c.tasks + ti);
+      const uint32_t n    = c.tasks[ti].n;
+      (void) ((n == (uint32_t) Ls ? (class_group<Ls, kBudget>(task, args, g, twSlot, heldSlot, colSlot), true) : false) ||
+              ...);
+   };
+   for (uint32_t it = 0;; ++it) {
+      // two slots: thread 0 writes slot it & 1 while no work item can still
+      // be reading it (they all passed the previous unit's barriers)
+      if (threadIdx.x == 0)
+         sItem[it & 1u] = __hip_atomic_fetch_add(ctr + kRqQueue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      const uint32_t w = sItem[it & 1u];
+      if (w >= c.numItems) break;
+      const RaggedUnit u = c.units[w];
+      for (uint32_t g = u.g0; g < u.g1; ++g) run(u.task, g);
+      if (u.task2 != kNoTask) {
+         __syncthreads();   // the column pass's stores before the row pass's loads
+         for (uint32_t h = u.h0; h < u.h1; ++h) run(u.task2, h);
+      }
+   }
+   // the last workgroup out (all others have taken their last unit) clears
+   // the queue head and the exit count
+   if (threadIdx.x == 0)
+      *sLast = __hip_atomic_fetch_add(ctr + kRqExit, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
+   __syncthreads();
+   if (*sLast && threadIdx.x < kRqCounters) __hip_atomic_store(ctr + threadIdx.x, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The class's data, twiddle and (ragged_coltw_lds) column-table slots, plus
+// 4 words: two queue-entry slots and the last-workgroup flag.
+template <int... Ls>
+constexpr size_t ragged_lds_bytes()
+{
+   return class_lds_base_bytes<Ls...>() + (ragged_coltw_lds() ? (kColTwSlots + 1u) * sizeof(float2) : 0u) +
+          4u * sizeof(uint32_t);
+}
+
+template <int... Ls>
+uint32_t ragged_resident_groups()
+{
+   static uint32_t cached = 0;
+   if (cached == 0) {
+      int dev = 0, cus = 0, per = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+         cus = 256;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_class_kernel<Ls...>, class_threads<Ls...>(),
+                                                       ragged_lds_bytes<Ls...>()) != hipSuccess ||
+          per <= 0)
+         per = 1;
+      cached = (uint32_t) cus * (uint32_t) per;
+   }
+   return cached;
+}
+
+// grid 0: the resident workgroups (a larger grid only adds workgroups that
+// find the queue empty).
+template <int... Ls>
+hipError_t launch_ragged_class(const ClassArgs &c, uint32_t grid, hipStream_t stream)
+{
+   if (c.numItems == 0) return hipSuccess;
+   if (c.counters == nullptr || c.units == nullptr || c.tasks == nullptr) return hipErrorInvalidValue;
+   constexpr size_t lds    = ragged_lds_bytes<Ls...>();
+   static bool      ldsSet = false;
+   if (lds > 65536 && !ldsSet) {
+      const hipError_t e = hipFuncSetAttribute((const void *) ragged_class_kernel<Ls...>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+      if (e != hipSuccess) return e;
+      ldsSet = true;
+   }
+   uint32_t target = grid != 0 ? grid : ragged_resident_groups<Ls...>();
+   target          = min(target, c.numItems);
+   hipLaunchKernelGGL((ragged_class_kernel<Ls...>), dim3(target), dim3(class_threads<Ls...>()), lds, stream, c);
    return hipGetLastError();
 }
 

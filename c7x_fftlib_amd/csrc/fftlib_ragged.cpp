@@ -11,8 +11,19 @@
 // FFTLIB_ifft1dBatched_i32fc_c32fc_o32fc (conj, x(1.0f/N), conj), with the
 // reference's twiddle tables.
 //
-// Plan: transforms are grouped by length. Lengths up to 16384 become tasks of
-// the multi-length kernel of their octave (fft_fixed_class.h). A longer length
+// Two forms of plan, chosen per plan (FFTLIB_GPU_raggedSetForm) or by the
+// autotuner:
+//
+// Persistent form (FFTLIB_RAGGED_PERSISTENT): one launch of the
+// persistent ragged kernel (fft_ragged_class.hip) per direction runs every
+// transform from a work queue (fftlib_ragged_queue.h). Lengths up to 4096 are
+// plain tasks, one queue unit per group; each transform of a longer length
+// n = n1 * m2 is one unit run by one workgroup: its column pass into its own
+// scratch region, a barrier, its row pass.
+//
+// Graph form (FFTLIB_RAGGED_GRAPH, the default): transforms are grouped by
+// length. Lengths up to 16384 become tasks of the multi-length kernel of
+// their octave (fft_fixed_class.h). A longer length
 // n = n1 * m2 runs in two passes through its own region of a plan-owned
 // scratch buffer: a column task (n1-point DFTs of its m2 columns, twiddled)
 // in the class kernel of n1, then a row task (m2-point DFTs of its n1 rows,
@@ -38,6 +49,7 @@
 #include "fft_fixed_impl.h"
 #include "fft_kernels.h"
 #include "fftlib_gpu.h"
+#include "fftlib_ragged_queue.h"
 #include "fftlib_ragged_schedule.h"
 #include "fftlib_runtime.h"
 
@@ -53,6 +65,10 @@ struct FFTLIB_GPU_ragged_s {
       float            *dTw      = nullptr; // twiddle table of length n (one-pass lengths)
       uint64_t          scrOff   = 0;       // two-pass: complex offset of its scratch region
       double            work     = 0.0;     // n * log2(n) * count, for ordering
+      // persistent form: lengths above kRaggedClassMaxN run two passes
+      bool              pTwoPass = false;
+      TwoPassPlan       pPlan{};
+      uint64_t          pScrOff  = 0;
    };
    std::vector<Group> groups;
    struct Class {
@@ -67,6 +83,14 @@ struct FFTLIB_GPU_ragged_s {
       double                  work     = 0.0;
    };
    std::vector<Class> classes;   // phase 0 first, then phase 1; by work within a phase
+   // persistent form
+   int                form        = FFTLIB_RAGGED_GRAPH;   // the faster form on the ragged 10000 workload
+   bool               formSet     = false;   // chosen by FFTLIB_GPU_raggedSetForm (the tuner keeps it)
+   fixedk::ClassTask  *pTasks     = nullptr;
+   fixedk::RaggedUnit *pUnits     = nullptr;
+   uint32_t           *pCounters  = nullptr;
+   uint32_t            pNumItems  = 0;   // units
+   uint32_t           pGrid       = 0;       // workgroups of the persistent launch (0: resident)
    uint64_t           totalPoints = 0;
    float2            *scratch     = nullptr;
    size_t             scratchSize = 0;
@@ -139,6 +163,15 @@ hipError_t launch_class(const FFTLIB_GPU_ragged_s &h, const FFTLIB_GPU_ragged_s:
    return fixed_class_launch(c.cls, a, grid, s);
 }
 
+hipError_t launch_persistent(const FFTLIB_GPU_ragged_s &h, const float *in, float *out, int mode, uint32_t grid,
+                             hipStream_t s)
+{
+   fixedk::ClassArgs a{};
+   a.x = (const float2 *) in, a.y = (float2 *) out, a.scratch = h.scratch, a.mode = mode;
+   a.numItems = h.pNumItems, a.tasks = h.pTasks, a.units = h.pUnits, a.counters = h.pCounters;
+   return ragged_class_launch(a, grid, s);
+}
+
 FFTLIB_STATUS fail(const char *what, hipError_t e)
 {
    set_last_error(what, e);
@@ -157,6 +190,9 @@ void release(FFTLIB_GPU_ragged_s *h)
       if (c.dTasks) (void) hipFree(c.dTasks);
       if (c.dItems) (void) hipFree(c.dItems);
    }
+   if (h->pTasks) (void) hipFree(h->pTasks);
+   if (h->pUnits) (void) hipFree(h->pUnits);
+   if (h->pCounters) (void) hipFree(h->pCounters);
    if (h->scratch) (void) hipFree(h->scratch);
    for (hipEvent_t e : h->events) (void) hipEventDestroy(e);
    for (hipStream_t s : h->lanes) (void) hipStreamDestroy(s);
@@ -199,6 +235,11 @@ struct Direction {
 hipError_t enqueue(FFTLIB_GPU_ragged_s *h, const Direction *dirs, int count, hipStream_t origin)
 {
    hipError_t e = hipSuccess;
+   if (h->form == FFTLIB_RAGGED_PERSISTENT) {   // one launch per direction, in stream order
+      for (int d = 0; d < count && e == hipSuccess; ++d)
+         e = launch_persistent(*h, dirs[d].in, dirs[d].out, dirs[d].mode, h->pGrid, origin);
+      return e;
+   }
    for (const ragged::Step &st : ragged::schedule(kinds_of(h), links_of(h), (int) h->lanes.size(), count)) {
       const hipStream_t s = st.stream == ragged::kOrigin ? origin : h->lanes[(size_t) st.stream];
       if (st.op == ragged::Op::Launch) {
@@ -311,6 +352,67 @@ hipError_t build_classes(FFTLIB_GPU_ragged_s *h)
    return hipSuccess;
 }
 
+// Tasks, units and counters of the persistent form: one plain task per
+// length up to kRaggedClassMaxN; a column and a row task per transform of a
+// longer length (count 1: its offset entry and its scratch region).
+hipError_t build_persistent(FFTLIB_GPU_ragged_s *h)
+{
+   std::vector<fixedk::ClassTask>  tasks;
+   std::vector<ragged::QueueEntry> q;
+   for (const auto &g : h->groups) {
+      if (!g.pTwoPass) {
+         fixedk::ClassTask k{};
+         k.n = g.n, k.numChannels = g.count, k.layout = fixedk::LAYOUT_PLAIN, k.tw = (const float2 *) g.dTw;
+         k.inOff = g.dOff, k.outOff = g.dOff;
+         k.numGroups = (k.numChannels + class_group_size(k.n) - 1u) / class_group_size(k.n);
+         q.push_back({false, (uint32_t) tasks.size(), k.numGroups, 0u, 0u, g.work});
+         tasks.push_back(k);
+         continue;
+      }
+      const uint64_t region = (uint64_t) g.pPlan.n1 * g.pPlan.s2;   // complex points per transform
+      const double   work   = g.work / (double) g.count;
+      for (uint32_t i = 0; i < g.count; ++i) {
+         const fixedk::ClassTask col =
+            two_pass_task(g.pPlan, fixedk::LAYOUT_COLUMNS, 1u, g.dOff + i, g.pScrOff + (uint64_t) i * region);
+         const fixedk::ClassTask row =
+            two_pass_task(g.pPlan, fixedk::LAYOUT_ROWS, 1u, g.dOff + i, g.pScrOff + (uint64_t) i * region);
+         q.push_back({true, (uint32_t) tasks.size(), col.numGroups, (uint32_t) tasks.size() + 1u, row.numGroups, work});
+         tasks.push_back(col);
+         tasks.push_back(row);
+      }
+   }
+   for (const auto &k : tasks)
+      if (k.n > kRaggedClassMaxN) return hipErrorInvalidValue;
+   const std::vector<ragged::QueueUnit> qu = ragged::build_units(q);
+   if (qu.empty() || qu.size() > 0xFFFFFFFFull - 65536u) return hipErrorInvalidValue;
+   std::vector<fixedk::RaggedUnit> units(qu.size());
+   for (size_t i = 0; i < qu.size(); ++i)
+      units[i] = fixedk::RaggedUnit{qu[i].task, qu[i].g0, qu[i].g1, qu[i].task2, qu[i].h0, qu[i].h1};
+   static_assert(ragged::kQueueNoTask == fixedk::kNoTask, "one marker for a unit without a second task");
+   hipError_t e;
+   if ((e = hipMalloc((void **) &h->pTasks, tasks.size() * sizeof(fixedk::ClassTask))) != hipSuccess ||
+       (e = hipMemcpy(h->pTasks, tasks.data(), tasks.size() * sizeof(fixedk::ClassTask), hipMemcpyHostToDevice)) !=
+          hipSuccess ||
+       (e = hipMalloc((void **) &h->pUnits, units.size() * sizeof(fixedk::RaggedUnit))) != hipSuccess ||
+       (e = hipMemcpy(h->pUnits, units.data(), units.size() * sizeof(fixedk::RaggedUnit), hipMemcpyHostToDevice)) !=
+          hipSuccess ||
+       (e = hipMalloc((void **) &h->pCounters, fixedk::kRqCounters * sizeof(uint32_t))) != hipSuccess ||
+       (e = hipMemset(h->pCounters, 0, fixedk::kRqCounters * sizeof(uint32_t))) != hipSuccess ||
+       (e = hipDeviceSynchronize()) != hipSuccess)
+      return e;
+   h->pNumItems = (uint32_t) units.size();
+   return hipSuccess;
+}
+
+// Tuning-cache keys of the persistent form: its grid (bucket = item count)
+// and the form the tuner chose for plans of a size (bucket = points / 1024).
+constexpr uint32_t kPersistentGridKey = 200000u;
+constexpr uint32_t kFormKey           = 200001u;
+uint32_t           form_count(const FFTLIB_GPU_ragged_s *h)
+{
+   return (uint32_t) std::min<uint64_t>(h->totalPoints >> 10, 1ull << 30);
+}
+
 }  // namespace
 
 extern "C" {
@@ -332,6 +434,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
    h->totalPoints = off;
    hipError_t e   = hipSuccess;
    size_t     scr = 0;
+
    for (auto &kv : byLen) {
       FFTLIB_GPU_ragged_s::Group g;
       g.n     = kv.first;
@@ -347,6 +450,17 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
          }
          g.scrOff = scr / sizeof(float2);
          scr += (two_pass_scratch_points(g.n, g.count) * 8u + 255u) & ~(size_t) 255u;
+      }
+      // the persistent form splits every length above kRaggedClassMaxN (the
+      // same split as the graph form's above 16384); its scratch regions
+      // follow the graph form's
+      g.pTwoPass = g.n > kRaggedClassMaxN;
+      if (g.pTwoPass) {
+         if ((e = two_pass_plan(g.n, &g.pPlan)) != hipSuccess) {
+            release(h);
+            delete h;
+            return fail("ragged two-pass tables", e);
+         }
       }
       if ((e = hipMalloc((void **) &g.dOff, kv.second.size() * sizeof(uint64_t))) == hipSuccess)
          e = hipMemcpy(g.dOff, kv.second.data(), kv.second.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
@@ -373,6 +487,26 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       release(h);
       delete h;
       return fail("ragged class tables", e);
+   }
+   // persistent form: its own scratch regions in the same buffer (a plan
+   // runs one form at a time), 256-byte aligned like the graph form's
+   size_t pscr = 0;
+   for (auto &g : h->groups)
+      if (g.pTwoPass) {
+         g.pScrOff = pscr / sizeof(float2);
+         pscr += ((size_t) g.pPlan.n1 * g.pPlan.s2 * g.count * 8u + 255u) & ~(size_t) 255u;
+      }
+   scr = std::max(scr, pscr);
+   if ((e = build_persistent(h)) != hipSuccess) {
+      release(h);
+      delete h;
+      return fail("ragged persistent tables", e);
+   }
+   {
+      uint32_t g = 0;
+      if (tuned(kPersistentGridKey, h->pNumItems, &g)) h->pGrid = g;
+      if (tuned(kFormKey, form_count(h), &g) && (g == FFTLIB_RAGGED_GRAPH || g == FFTLIB_RAGGED_PERSISTENT))
+         h->form = (int) g;
    }
    if (scr) {
       if ((e = hipMalloc((void **) &h->scratch, scr)) != hipSuccess) {
@@ -492,6 +626,53 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
       std::lock_guard<std::mutex> tl(g_tuneMu);
       g_tune[TuneKey{class_key(c), bucket_of(c.numItems)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
    }
+   // one forward direction of a form, eager, 3 timed after 1 warm (us)
+   const int savedForm  = handle->form;
+   auto      time_form = [&](int form, uint32_t grid, float *us) {
+      handle->form = form;
+      const uint32_t savedGrid = handle->pGrid;
+      handle->pGrid            = grid;
+      const Direction fwd{pX, pY, C2C_FORWARD};
+      hipError_t      r = enqueue(handle, &fwd, 1, s);
+      float           ms = 0.0f;
+      if (r == hipSuccess) r = hipEventRecord(t0, s);
+      for (int k = 0; k < 3 && r == hipSuccess; ++k) r = enqueue(handle, &fwd, 1, s);
+      if (r == hipSuccess) r = hipEventRecord(t1, s);
+      if (r == hipSuccess) r = hipEventSynchronize(t1);
+      if (r == hipSuccess) r = hipEventElapsedTime(&ms, t0, t1);
+      handle->pGrid = savedGrid;
+      handle->form  = savedForm;
+      *us           = ms * 1000.0f / 3.0f;
+      return r;
+   };
+   // persistent form: grid candidates (0 = the resident workgroups)
+   float persistUs = 1e30f;
+   if (e == hipSuccess) {
+      const uint32_t res    = ragged_class_resident();
+      const uint32_t cand[] = {0u, res * 3u / 4u, res / 2u, res / 4u};
+      uint32_t       bestG  = 0;
+      for (uint32_t g : cand) {
+         float us = 0.0f;
+         if ((e = time_form(FFTLIB_RAGGED_PERSISTENT, g, &us)) != hipSuccess) break;
+         if (us < persistUs) persistUs = us, bestG = g;
+      }
+      if (e == hipSuccess) {
+         handle->pGrid = bestG;
+         std::lock_guard<std::mutex> tl(g_tuneMu);
+         g_tune[TuneKey{kPersistentGridKey, bucket_of(handle->pNumItems)}] = TuneVal{bestG, persistUs};
+      }
+   }
+   // the form: the faster one, unless the caller chose it
+   if (e == hipSuccess) {
+      float graphUs = 0.0f;
+      if ((e = time_form(FFTLIB_RAGGED_GRAPH, 0u, &graphUs)) == hipSuccess) {
+         const int form = graphUs < persistUs ? FFTLIB_RAGGED_GRAPH : FFTLIB_RAGGED_PERSISTENT;
+         if (!handle->formSet) handle->form = form;
+         std::lock_guard<std::mutex> tl(g_tuneMu);
+         g_tune[TuneKey{kFormKey, bucket_of(form_count(handle))}] =
+            TuneVal{(uint32_t) form, std::min(graphUs, persistUs)};
+      }
+   }
    (void) hipEventDestroy(t0);
    (void) hipEventDestroy(t1);
    if (e != hipSuccess) return fail("ragged tuning", e);
@@ -500,6 +681,22 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    if (handle->graph) (void) hipGraphDestroy(handle->graph), handle->graph = nullptr;
    return FFTLIB_SUCCESS;
 }
+
+FFTLIB_STATUS FFTLIB_GPU_raggedSetForm(FFTLIB_GPU_raggedHandle handle, int32_t form)
+{
+   if (handle == nullptr) return FFTLIB_ERR_NULL_POINTER;
+   if (form != FFTLIB_RAGGED_GRAPH && form != FFTLIB_RAGGED_PERSISTENT) return FFTLIB_ERR_INVALID_VALUE;
+   std::lock_guard<std::mutex> lock(handle->mu);
+   if (handle->form != form) {   // the captured graph (if any) is of the other form
+      if (handle->exec) (void) hipGraphExecDestroy(handle->exec), handle->exec = nullptr;
+      if (handle->graph) (void) hipGraphDestroy(handle->graph), handle->graph = nullptr;
+   }
+   handle->form    = form;
+   handle->formSet = true;
+   return FFTLIB_SUCCESS;
+}
+
+int32_t FFTLIB_GPU_raggedGetForm(FFTLIB_GPU_raggedHandle handle) { return handle ? handle->form : -1; }
 
 FFTLIB_STATUS FFTLIB_GPU_tuneSave(const char *path)
 {

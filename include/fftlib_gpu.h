@@ -91,8 +91,24 @@ int32_t       FFTLIB_GPU_getC2CVariant(void);
  * phase 0 = lengths up to 16384 and the column pass of longer lengths,
  * phase 1 = the row pass of longer lengths).
  *
+ * Two forms of plan (FFTLIB_GPU_raggedSetForm; FFTLIB_GPU_raggedTune times
+ * both and keeps the faster unless the caller chose one):
+ *   FFTLIB_RAGGED_PERSISTENT: one persistent kernel launch per
+ *     direction runs every transform from a work queue; each transform
+ *     longer than 4096 points is one unit of the queue, run by one workgroup
+ *     as a column pass into its own scratch region and a row pass after a
+ *     workgroup barrier (no unit waits for another).
+ *   FFTLIB_RAGGED_GRAPH (default): one multi-length kernel launch per octave
+ *     of lengths and pass, forked over several streams and joined per pass.
+ * The tuning cache also holds the persistent launch's workgroup count (key
+ * 200000) and the chosen form for plans of a size (key 200001, "grid" =
+ * the form, bucket = total points / 1024).
+ *
  * A plan owns device memory: offset and twiddle tables per length, a scratch
- * buffer of 8 bytes per point of the lengths above 16384, and its graph. */
+ * buffer of 8 bytes per point of the lengths above 4096, counters of the
+ * persistent launch, and its graph. */
+#define FFTLIB_RAGGED_GRAPH      0
+#define FFTLIB_RAGGED_PERSISTENT 1
 typedef struct FFTLIB_GPU_ragged_s *FFTLIB_GPU_raggedHandle;
 FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uint32_t *lengths, uint32_t count);
 FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB_F32 *pX, FFTLIB_F32 *pY,
@@ -100,6 +116,8 @@ FFTLIB_STATUS FFTLIB_GPU_raggedExec(FFTLIB_GPU_raggedHandle handle, const FFTLIB
 FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB_F32 *pX, FFTLIB_F32 *pY);
 FFTLIB_STATUS FFTLIB_GPU_raggedDestroy(FFTLIB_GPU_raggedHandle handle);
 uint64_t      FFTLIB_GPU_raggedPoints(FFTLIB_GPU_raggedHandle handle);
+FFTLIB_STATUS FFTLIB_GPU_raggedSetForm(FFTLIB_GPU_raggedHandle handle, int32_t form);
+int32_t       FFTLIB_GPU_raggedGetForm(FFTLIB_GPU_raggedHandle handle);
 FFTLIB_STATUS FFTLIB_GPU_tuneSave(const char *path);
 FFTLIB_STATUS FFTLIB_GPU_tuneLoad(const char *path);
 

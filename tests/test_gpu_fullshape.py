@@ -222,12 +222,13 @@ def test_r2c_c2r_4096_x16384():
 
 
 # ---- config: the ragged 10000-transform batch, forward and inverse ------------
-def test_ragged_10000_forward_inverse():
+@pytest.mark.parametrize("form", [1, 0])   # persistent launch, graph of per-octave launches
+def test_ragged_10000_forward_inverse(form):
     import bench
     torch = _torch()
     lengths = bench.Workload.ragged_lengths(10000, 10000)
     total = int(lengths.sum())
-    print(f"[fullshape] ragged 10000 ({total} points)", flush=True)
+    print(f"[fullshape] ragged 10000 ({total} points), form {form}", flush=True)
     L = G.lib()
     assert L.FFTLIB_setFunctionStyle(F.FFTLIB_FUNCTION_GPU) == 0
     L.FFTLIB_GPU_setSynchronous(1)
@@ -235,6 +236,7 @@ def test_ragged_10000_forward_inverse():
     h = ctypes.c_void_p()
     assert L.FFTLIB_GPU_raggedCreate(ctypes.byref(h), arr, len(lengths)) == 0, G.F.last_error()
     try:
+        assert L.FFTLIB_GPU_raggedSetForm(h, form) == 0
         x = host_cplx(10001, 1, total)[0]
         dx = torch.from_numpy(x.view(np.float32)).cuda()
         dy = torch.full_like(dx, float("nan"))
