@@ -6,12 +6,21 @@
 // src/linalg_c7xmma/FFTLIB_LINALG_matrixMatrixMultiply_ixX_ixX_oxX/..._cn.cpp:27-65,
 // src/fft_c7xmma/FFTLIB_FFT_highRadixDecompositions_ixX_cxX_oxX/..._cn.cpp:19-455).
 //
-// These kernels are small (the reference's sizes: up to 8192 points, 2048
-// channels) and are bound by launch latency rather than by HBM or ALU: each
-// is one work item per complex output, computing its dot products with exact
-// integer arithmetic (v_mad_i64_i32 for INT16 data, 128-bit adds for INT32).
-// The MFMA integer path (int8 operands) would need the data split into four
-// byte planes per INT32 value and is not used.
+// The DFT-matrix products of dftSmall and dftLarge (Y = X W over the batch)
+// run on the FP64 matrix cores (v_mfma_f64_16x16x4f64), exactly: every
+// product of two INT16 values (|p| <= 2^30) and every sum of up to 2^22 of
+// them is an integer below 2^53, so the FP64 accumulation is the exact
+// integer sum. INT32 values are split into a signed high and an unsigned low
+// 16-bit part, x = xh 2^16 + xl; the three accumulators sum xh wh, xh wl +
+// xl wh and xl wl (each term below 2^32, each sum below 2^53 for the K of
+// these kernels), and the 128-bit accumulator is rebuilt from them exactly.
+// The rounding shift and saturation follow as in the reference.
+//
+// The stages of highRadixDecompositions are the same kind of product (a small
+// DFT matrix times many complex vectors, with the reference's saturating
+// negations applied to the gathered operands) on the same exact tiles. These
+// kernels are small (the reference's sizes: up to 8192 points, 2048
+// channels) and bound by launch latency rather than by HBM or ALU.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -76,19 +85,111 @@ __device__ __forceinline__ typename Acc<T>::type mul(T a, T b)
    return (typename Acc<T>::type) ((int64_t) a * (int64_t) b);
 }
 
-// Y[l][j] = shift_round(sum_k X[l][k] W[k*ldw + j]), l < rows, j, k < k2.
-template <typename T>
-__global__ void __launch_bounds__(kThreads) matrix_kernel(const T *__restrict__ x, const T *__restrict__ w,
-                                                          T *__restrict__ y, uint32_t rows, uint32_t k2,
-                                                          uint32_t ldw, uint32_t shift)
+// ---- DFT-matrix product on the FP64 matrix cores --------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c)
 {
-   const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
-   if (t >= rows * k2) return;
-   const uint32_t l = t / k2, j = t - l * k2;
-   const T       *xr = x + (size_t) l * k2;
-   typename Acc<T>::type acc = 0;
-   for (uint32_t k = 0; k < k2; ++k) acc += mul(xr[k], w[(size_t) ldw * k + j]);
-   y[(size_t) l * k2 + j] = shift_round<T>(acc, shift);
+   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Largest K (dot-product length) whose exact sum the FP64 accumulators hold
+// (INT32 parts: |term| < 2^32, two terms per K in the middle accumulator).
+constexpr uint32_t kMfmaMaxK = 1u << 20;
+
+// One 16 x 16 tile of Y = X W per wavefront: operand layouts of
+// v_mfma_f64_16x16x4f64 for lane l = 16 q + r (checked on the GPU with a
+// one-hot probe): A[r][q] (row r, k = q of the step), B[q][r] (k = q,
+// column r); element v of the result is D[q + 4 v][r].
+template <typename T>
+__global__ void __launch_bounds__(kThreads) matrix_mfma_kernel(const T *__restrict__ x, const T *__restrict__ w,
+                                                               T *__restrict__ y, uint32_t rows, uint32_t k2,
+                                                               uint32_t ldw, uint32_t shift)
+{
+   const uint32_t lane = threadIdx.x % 64u, q = lane / 16u, r = lane % 16u;
+   const uint32_t tilesJ = (k2 + 15u) / 16u;
+   const uint32_t tile   = blockIdx.x * (kThreads / 64u) + threadIdx.x / 64u;
+   const uint32_t ti = tile / tilesJ, tj = tile - ti * tilesJ;
+   if (ti * 16u >= rows) return;   // the whole wavefront
+   const uint32_t l = ti * 16u + r, j = tj * 16u + r;
+   f64x4          acc[3];
+#pragma unroll
+   for (int a = 0; a < 3; ++a) acc[a] = f64x4{0.0, 0.0, 0.0, 0.0};
+   for (uint32_t k0 = 0; k0 < k2; k0 += 4u) {
+      const uint32_t k  = k0 + q;
+      const int32_t  av = (l < rows && k < k2) ? (int32_t) x[(size_t) l * k2 + k] : 0;
+      const int32_t  bv = (j < k2 && k < k2) ? (int32_t) w[(size_t) k * ldw + j] : 0;
+      if constexpr (sizeof(T) == 2) {
+         acc[0] = mfma_f64((double) av, (double) bv, acc[0]);
+      }
+      else {
+         const double ah = (double) (av >> 16), al = (double) (av & 0xFFFF);
+         const double bh = (double) (bv >> 16), bl = (double) (bv & 0xFFFF);
+         acc[0] = mfma_f64(ah, bh, acc[0]);
+         acc[1] = mfma_f64(ah, bl, acc[1]);
+         acc[1] = mfma_f64(al, bh, acc[1]);
+         acc[2] = mfma_f64(al, bl, acc[2]);
+      }
+   }
+#pragma unroll
+   for (int v = 0; v < 4; ++v) {
+      const uint32_t row = ti * 16u + q + 4u * (uint32_t) v, col = tj * 16u + r;
+      if (row >= rows || col >= k2) continue;
+      typename Acc<T>::type s;
+      if constexpr (sizeof(T) == 2) {
+         s = (int64_t) acc[0][v];
+      }
+      else {
+         s = ((__int128) (int64_t) acc[0][v] << 32) + ((__int128) (int64_t) acc[1][v] << 16) + (__int128) (int64_t) acc[2][v];
+      }
+      y[(size_t) row * k2 + col] = shift_round<T>(s, shift);
+   }
+}
+
+// Exact integer sums over K terms of a(k) b(k) on the FP64 matrix cores for
+// one 16 x 16 tile: a(k) is the lane's A operand (its row r = lane % 16 at k),
+// b(k) its B operand (its column r at k); element v of the result holds
+// row q + 4 v, column r (q = lane / 16). INT32 operands are split as in
+// matrix_mfma_kernel.
+template <typename T>
+struct ExactTile {
+   f64x4 p[sizeof(T) == 2 ? 1 : 3];
+};
+
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ ExactTile<T> exact_tile(uint32_t K, FA a, FB b)
+{
+   const uint32_t q = (threadIdx.x % 64u) / 16u;
+   ExactTile<T>   t;
+#pragma unroll
+   for (auto &x : t.p) x = f64x4{0.0, 0.0, 0.0, 0.0};
+   for (uint32_t k0 = 0; k0 < K; k0 += 4u) {
+      const uint32_t k  = k0 + q;
+      const int32_t  av = k < K ? a(k) : 0;
+      const int32_t  bv = k < K ? b(k) : 0;
+      if constexpr (sizeof(T) == 2) {
+         t.p[0] = mfma_f64((double) av, (double) bv, t.p[0]);
+      }
+      else {
+         const double ah = (double) (av >> 16), al = (double) (av & 0xFFFF);
+         const double bh = (double) (bv >> 16), bl = (double) (bv & 0xFFFF);
+         t.p[0] = mfma_f64(ah, bh, t.p[0]);
+         t.p[1] = mfma_f64(ah, bl, t.p[1]);
+         t.p[1] = mfma_f64(al, bh, t.p[1]);
+         t.p[2] = mfma_f64(al, bl, t.p[2]);
+      }
+   }
+   return t;
+}
+
+template <typename T>
+__device__ __forceinline__ typename Acc<T>::type exact_value(const ExactTile<T> &t, int v)
+{
+   if constexpr (sizeof(T) == 2)
+      return (int64_t) t.p[0][v];
+   else
+      return ((__int128) (int64_t) t.p[0][v] << 32) + ((__int128) (int64_t) t.p[1][v] << 16) +
+             (__int128) (int64_t) t.p[2][v];
 }
 
 // Re-layout between non-interleaved (R0..R(N-1) I0..I(N-1)) and interleaved
@@ -121,123 +222,173 @@ __device__ __forceinline__ void twiddle_store(T *dst, size_t o, T ra, T ia, T w0
    dst[o + 1] = shift_round<T>(mul(neg(w0), ia) + mul(w1, ra), nm);
 }
 
-// Stage 0 (..._cn.cpp:76-128): the d0-point DFT of points p + m*(N/d0),
-// then the twiddle. One work item per (channel j, column group n, k, l).
+// ---- the high-radix stages as matrix products on the FP64 matrix cores --------
+// Each stage is a small DFT matrix times many complex vectors (the stages of
+// ..._cn.cpp): rows of the product are the stage's independent vectors,
+// columns its DFT outputs, K the real and imaginary parts of the inputs. The
+// reference's saturating negations are applied to the gathered operands, and
+// each output is rounded by the stage shift, multiplied by its twiddle and
+// rounded again (twiddle_store). One 16-row tile per wavefront.
+
+// Stage 0 (..._cn.cpp:76-128): the d0-point DFT of points p + m (N / d0),
+// then the twiddle. Rows (j, n, k), k < MMA/2; columns l < d0; K = 2 d0.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) hr_stage0_kernel(const T *__restrict__ src, T *__restrict__ dst,
-                                                             const T *__restrict__ w1, const T *__restrict__ w2,
-                                                             uint32_t N, uint32_t B, uint32_t d0, uint32_t shift)
+__global__ void __launch_bounds__(kThreads) hr_stage0_mfma(const T *__restrict__ src, T *__restrict__ dst,
+                                                           const T *__restrict__ w1, const T *__restrict__ w2,
+                                                           uint32_t N, uint32_t B, uint32_t d0, uint32_t shift)
 {
    constexpr uint32_t MMA = Acc<T>::mma;
-   using A                = typename Acc<T>::type;
+   const uint32_t lane = threadIdx.x % 64u, q = lane / 16u, r = lane % 16u;
    const uint32_t nGroups = (N * 2) / (MMA * d0);
-   const uint32_t t       = blockIdx.x * kThreads + threadIdx.x;
-   if (t >= B * nGroups * (MMA / 2) * d0) return;
-   uint32_t       r = t;
-   const uint32_t l = r % d0;
-   r /= d0;
-   const uint32_t k = r % (MMA / 2);
-   r /= MMA / 2;
-   const uint32_t n = r % nGroups;
-   const uint32_t j = r / nGroups;
-   A              re = 0, im = 0;
-   for (uint32_t m = 0; m < d0; ++m) {
-      const size_t b  = (size_t) j * N * 2 + (n * MMA + k * 2) + (m * N * 2) / d0;
-      const T      c  = w1[l * MMA + m * 2], s = w1[l * MMA + m * 2 + 1];
-      im += mul(c, src[b + 1]);
-      im += mul(s, src[b]);
-      re += mul(c, src[b]);
-      re += mul(s, neg(src[b + 1]));
+   const uint32_t rows    = B * nGroups * (MMA / 2);
+   const uint32_t tile    = blockIdx.x * (kThreads / 64u) + threadIdx.x / 64u;
+   if (tile * 16u >= rows) return;
+   auto base = [&](uint32_t R) {
+      const uint32_t k = R % (MMA / 2), n = (R / (MMA / 2)) % nGroups, j = R / (MMA / 2) / nGroups;
+      return (size_t) j * N * 2 + (n * MMA + k * 2);
+   };
+   const uint32_t Ra = tile * 16u + r;
+   const bool     va = Ra < rows;
+   const size_t   ba = va ? base(Ra) : 0;
+   auto           at = [&](uint32_t kk) { return ba + ((kk >> 1) * N * 2) / d0; };
+   auto           wb = [&](uint32_t kk) -> int32_t { return r < d0 ? w1[r * MMA + kk] : 0; };
+   const ExactTile<T> re = exact_tile<T>(2 * d0, [&](uint32_t kk) -> int32_t {
+      if (!va) return 0;
+      const size_t b = at(kk);
+      return (kk & 1u) ? neg(src[b + 1]) : src[b];
+   }, wb);
+   const ExactTile<T> im = exact_tile<T>(2 * d0, [&](uint32_t kk) -> int32_t {
+      if (!va) return 0;
+      const size_t b = at(kk);
+      return (kk & 1u) ? src[b] : src[b + 1];
+   }, wb);
+#pragma unroll
+   for (int v = 0; v < 4; ++v) {
+      const uint32_t R = tile * 16u + q + 4u * (uint32_t) v, l = r;
+      if (R >= rows || l >= d0) continue;
+      const uint32_t k = R % (MMA / 2), n = (R / (MMA / 2)) % nGroups, j = R / (MMA / 2) / nGroups;
+      const T        ra = shift_round<T>(exact_value(re, v), shift), ia = shift_round<T>(exact_value(im, v), shift);
+      const uint32_t wi = n * MMA * d0 + k * 2 + l * MMA;
+      twiddle_store(dst, (size_t) j * N * 2 + (n * MMA + k * 2) + (l * N * 2) / d0, ra, ia, w2[wi], w2[wi + 1]);
    }
-   const T        ra = shift_round<T>(re, shift), ia = shift_round<T>(im, shift);
-   const uint32_t wi = n * MMA * d0 + k * 2 + l * MMA;
-   twiddle_store(dst, (size_t) j * N * 2 + (n * MMA + k * 2) + (l * N * 2) / d0, ra, ia, w2[wi], w2[wi + 1]);
 }
 
 // Stages 1-3 (..._cn.cpp:138-203): the MMA/2-point DFT over points spaced by
-// d/(MMA/2) within sub-transforms of d points, then the twiddle.
+// d / (MMA/2) within sub-transforms of d points, then the twiddle. Rows
+// (j, k, l, i), i < MMA/2; columns m < MMA/2; K = MMA.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) hr_stage_kernel(const T *__restrict__ src, T *__restrict__ dst,
-                                                            const T *__restrict__ w1, const T *__restrict__ w2,
-                                                            uint32_t N, uint32_t B, uint32_t d, uint32_t shift)
+__global__ void __launch_bounds__(kThreads) hr_stage_mfma(const T *__restrict__ src, T *__restrict__ dst,
+                                                          const T *__restrict__ w1, const T *__restrict__ w2,
+                                                          uint32_t N, uint32_t B, uint32_t d, uint32_t shift)
 {
    constexpr uint32_t MMA = Acc<T>::mma;
-   using A                = typename Acc<T>::type;
-   const uint32_t nSub    = N / d;
-   const uint32_t nL      = (d * 4) / (MMA * MMA);
-   const uint32_t t       = blockIdx.x * kThreads + threadIdx.x;
-   if (t >= B * nSub * nL * (MMA / 2) * (MMA / 2)) return;
-   uint32_t       r = t;
-   const uint32_t m = r % (MMA / 2);
-   r /= MMA / 2;
-   const uint32_t i = r % (MMA / 2);
-   r /= MMA / 2;
-   const uint32_t l = r % nL;
-   r /= nL;
-   const uint32_t k = r % nSub;
-   const uint32_t j = r / nSub;
-   A              re = 0, im = 0;
-   for (uint32_t n = 0; n < MMA / 2; ++n) {
-      const size_t b = (size_t) j * N * 2 + k * d * 2 + (l * MMA + i * 2) + n * (d * 4) / MMA;
-      const T      c = w1[m * MMA + n * 2], s = w1[m * MMA + n * 2 + 1];
-      im += mul(c, src[b]);
-      im += mul(s, src[b + 1]);
-      re += mul(c, src[b + 1]);
-      re += mul(s, neg(src[b]));
+   const uint32_t lane = threadIdx.x % 64u, q = lane / 16u, r = lane % 16u;
+   const uint32_t nSub = N / d, nL = (d * 4) / (MMA * MMA);
+   const uint32_t rows = B * nSub * nL * (MMA / 2);
+   const uint32_t tile = blockIdx.x * (kThreads / 64u) + threadIdx.x / 64u;
+   if (tile * 16u >= rows) return;
+   auto base = [&](uint32_t R) {
+      const uint32_t i = R % (MMA / 2), l = (R / (MMA / 2)) % nL, k = (R / (MMA / 2) / nL) % nSub,
+                     j = R / (MMA / 2) / nL / nSub;
+      return (size_t) j * N * 2 + k * d * 2 + (l * MMA + i * 2);
+   };
+   const uint32_t Ra = tile * 16u + r;
+   const bool     va = Ra < rows;
+   const size_t   ba = va ? base(Ra) : 0;
+   auto           at = [&](uint32_t kk) { return ba + (kk >> 1) * (d * 4) / MMA; };
+   auto           wb = [&](uint32_t kk) -> int32_t { return r < MMA / 2 ? w1[r * MMA + kk] : 0; };
+   const ExactTile<T> re = exact_tile<T>(MMA, [&](uint32_t kk) -> int32_t {
+      if (!va) return 0;
+      const size_t b = at(kk);
+      return (kk & 1u) ? neg(src[b]) : src[b + 1];
+   }, wb);
+   const ExactTile<T> im = exact_tile<T>(MMA, [&](uint32_t kk) -> int32_t {
+      if (!va) return 0;
+      const size_t b = at(kk);
+      return (kk & 1u) ? src[b + 1] : src[b];
+   }, wb);
+#pragma unroll
+   for (int v = 0; v < 4; ++v) {
+      const uint32_t R = tile * 16u + q + 4u * (uint32_t) v, m = r;
+      if (R >= rows || m >= MMA / 2) continue;
+      const uint32_t i = R % (MMA / 2), l = (R / (MMA / 2)) % nL, k = (R / (MMA / 2) / nL) % nSub,
+                     j = R / (MMA / 2) / nL / nSub;
+      const T        ra = shift_round<T>(exact_value(re, v), shift), ia = shift_round<T>(exact_value(im, v), shift);
+      const uint32_t wi = l * MMA * MMA / 2 + i * 2 + m * MMA;
+      twiddle_store(dst, (size_t) j * N * 2 + k * d * 2 + l * MMA + i * 2 + m * (d * 4) / MMA, ra, ia, w2[wi],
+                    w2[wi + 1]);
    }
-   const T        ra = shift_round<T>(re, shift), ia = shift_round<T>(im, shift);
-   const uint32_t wi = l * MMA * MMA / 2 + i * 2 + m * MMA;
-   twiddle_store(dst, (size_t) j * N * 2 + k * d * 2 + l * MMA + i * 2 + m * (d * 4) / MMA, ra, ia, w2[wi],
-                 w2[wi + 1]);
 }
 
 // Final stage (..._cn.cpp:383-433): MMA/2-point DFTs of consecutive points,
-// stored in digit-reversed (natural output) order.
+// stored in digit-reversed (natural output) order. Rows (j, k, m, i, h);
+// columns l < MMA/2; K = MMA.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) hr_dft_kernel(const T *__restrict__ src, T *__restrict__ dst,
-                                                          const T *__restrict__ wd, uint32_t N, uint32_t B,
-                                                          uint32_t d0, uint32_t mC, uint32_t iC, uint32_t hC,
-                                                          uint32_t shift)
+__global__ void __launch_bounds__(kThreads) hr_dft_mfma(const T *__restrict__ src, T *__restrict__ dst,
+                                                        const T *__restrict__ wd, uint32_t N, uint32_t B, uint32_t d0,
+                                                        uint32_t mC, uint32_t iC, uint32_t hC, uint32_t shift)
 {
    constexpr uint32_t MMA = Acc<T>::mma;
-   using A                = typename Acc<T>::type;
-   const uint32_t t       = blockIdx.x * kThreads + threadIdx.x;
-   if (t >= B * d0 * mC * iC * hC * (MMA / 2)) return;
-   uint32_t       r = t;
-   const uint32_t l = r % (MMA / 2);
-   r /= MMA / 2;
-   const uint32_t h = r % hC;
-   r /= hC;
-   const uint32_t i = r % iC;
-   r /= iC;
-   const uint32_t m = r % mC;
-   r /= mC;
-   const uint32_t k = r % d0;
-   const uint32_t j = r / d0;
-   A              re = 0, im = 0;
-   for (uint32_t n = 0; n < MMA / 2; ++n) {
-      const size_t b = (size_t) j * N * 2 + k * (N * 2) / d0 + m * (N * 4) / (d0 * MMA) +
-                       i * (N * 8) / (d0 * MMA * MMA) + h * MMA + n * 2;
-      re += mul(wd[l * 2 * MMA + n * 2], src[b]);
-      re += mul(wd[l * 2 * MMA + n * 2 + 1], src[b + 1]);
-      im += mul(wd[(l * 2 + 1) * MMA + n * 2], src[b]);
-      im += mul(wd[(l * 2 + 1) * MMA + n * 2 + 1], src[b + 1]);
+   const uint32_t lane = threadIdx.x % 64u, q = lane / 16u, r = lane % 16u;
+   const uint32_t rows = B * d0 * mC * iC * hC;
+   const uint32_t tile = blockIdx.x * (kThreads / 64u) + threadIdx.x / 64u;
+   if (tile * 16u >= rows) return;
+   const uint32_t Ra = tile * 16u + r;
+   const bool     va = Ra < rows;
+   size_t         ba = 0;
+   if (va) {
+      uint32_t t = Ra;
+      const uint32_t h = t % hC;
+      t /= hC;
+      const uint32_t i = t % iC;
+      t /= iC;
+      const uint32_t m = t % mC;
+      t /= mC;
+      const uint32_t k = t % d0, j = t / d0;
+      ba = (size_t) j * N * 2 + k * (N * 2) / d0 + m * (N * 4) / (d0 * MMA) + i * (N * 8) / (d0 * MMA * MMA) + h * MMA;
    }
-   const size_t o = (size_t) j * N * 2 + l * d0 * mC * iC * hC * 2 + h * d0 * mC * iC * 2 + i * d0 * mC * 2 +
-                    m * d0 * 2 + k * 2;
-   dst[o]     = shift_round<T>(re, shift);
-   dst[o + 1] = shift_round<T>(im, shift);
+   auto a = [&](uint32_t kk) -> int32_t { return va ? src[ba + kk] : 0; };
+   const ExactTile<T> re =
+      exact_tile<T>(MMA, a, [&](uint32_t kk) -> int32_t { return r < MMA / 2 ? wd[r * 2 * MMA + kk] : 0; });
+   const ExactTile<T> im =
+      exact_tile<T>(MMA, a, [&](uint32_t kk) -> int32_t { return r < MMA / 2 ? wd[(r * 2 + 1) * MMA + kk] : 0; });
+#pragma unroll
+   for (int v = 0; v < 4; ++v) {
+      const uint32_t R = tile * 16u + q + 4u * (uint32_t) v, l = r;
+      if (R >= rows || l >= MMA / 2) continue;
+      uint32_t t = R;
+      const uint32_t h = t % hC;
+      t /= hC;
+      const uint32_t i = t % iC;
+      t /= iC;
+      const uint32_t m = t % mC;
+      t /= mC;
+      const uint32_t k = t % d0, j = t / d0;
+      const size_t   o = (size_t) j * N * 2 + l * d0 * mC * iC * hC * 2 + h * d0 * mC * iC * 2 + i * d0 * mC * 2 +
+                       m * d0 * 2 + k * 2;
+      dst[o]     = shift_round<T>(exact_value(re, v), shift);
+      dst[o + 1] = shift_round<T>(exact_value(im, v), shift);
+   }
 }
 
 inline dim3 grid_of(uint64_t items) { return dim3((unsigned) ((items + kThreads - 1) / kThreads)); }
+// One wavefront per 16 rows.
+inline dim3 grid_of_tiles(uint64_t rows)
+{
+   const uint64_t tiles = (rows + 15u) / 16u;
+   return dim3((unsigned) ((tiles + kThreads / 64 - 1) / (kThreads / 64)));
+}
 
 template <typename T>
 hipError_t matrix_t(const void *x, const void *w, void *y, uint32_t rows, uint32_t k2, uint32_t ldw, uint32_t shift,
                     hipStream_t s)
 {
    if ((uint64_t) rows * k2 == 0) return hipSuccess;
-   hipLaunchKernelGGL(matrix_kernel<T>, grid_of((uint64_t) rows * k2), dim3(kThreads), 0, s, (const T *) x,
-                      (const T *) w, (T *) y, rows, k2, ldw, shift);
+   if (k2 > kMfmaMaxK) return hipErrorInvalidValue;   // beyond the exact FP64 sums (far above the API's sizes)
+   // one wavefront per 16 x 16 output tile
+   const uint64_t tiles = (uint64_t) ((rows + 15u) / 16u) * ((k2 + 15u) / 16u);
+   hipLaunchKernelGGL(matrix_mfma_kernel<T>, dim3((unsigned) ((tiles + kThreads / 64 - 1) / (kThreads / 64))),
+                      dim3(kThreads), 0, s, (const T *) x, (const T *) w, (T *) y, rows, k2, ldw, shift);
    return hipGetLastError();
 }
 
@@ -260,8 +411,8 @@ hipError_t high_radix_t(void *xv, const void *w1v, const void *w2v, void *yv, ui
    while (d0 > MMA / 2) d0 /= MMA / 2;
    uint32_t d = N, cnt = 0;
    const T *p2 = w2;
-   hipLaunchKernelGGL(hr_stage0_kernel<T>, grid_of((uint64_t) N * B), dim3(kThreads), 0, s, (const T *) src, dst, w1,
-                      p2, N, B, d0, shifts[0]);
+   hipLaunchKernelGGL(hr_stage0_mfma<T>, grid_of_tiles((uint64_t) B * ((N * 2) / (MMA * d0)) * (MMA / 2)),
+                      dim3(kThreads), 0, s, (const T *) src, dst, w1, p2, N, B, d0, shifts[0]);
    if ((e = hipGetLastError()) != hipSuccess) return e;
    T *tmp = src;
    src = dst, dst = tmp;
@@ -269,8 +420,9 @@ hipError_t high_radix_t(void *xv, const void *w1v, const void *w2v, void *yv, ui
    d /= d0;
    ++cnt;
    for (uint32_t st = 1; st <= 3 && d > MMA / 2; ++st) {
-      hipLaunchKernelGGL(hr_stage_kernel<T>, grid_of((uint64_t) N * B), dim3(kThreads), 0, s, (const T *) src, dst,
-                         w1 + MMA * MMA / 2, p2, N, B, d, shifts[st]);
+      hipLaunchKernelGGL(hr_stage_mfma<T>,
+                         grid_of_tiles((uint64_t) B * (N / d) * ((d * 4) / (MMA * MMA)) * (MMA / 2)), dim3(kThreads),
+                         0, s, (const T *) src, dst, w1 + MMA * MMA / 2, p2, N, B, d, shifts[st]);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       tmp = src, src = dst, dst = tmp;
       p2 += d * 2;
@@ -281,7 +433,7 @@ hipError_t high_radix_t(void *xv, const void *w1v, const void *w2v, void *yv, ui
    const uint32_t mC = q > MMA / 2 ? MMA / 2 : q;
    const uint32_t iC = q > MMA / 2 ? (r > MMA / 2 ? MMA / 2 : r) : 1;
    const uint32_t hC = r > MMA / 2 ? (N * 8) / (MMA * MMA * MMA * d0) : 1;
-   hipLaunchKernelGGL(hr_dft_kernel<T>, grid_of((uint64_t) B * d0 * mC * iC * hC * (MMA / 2)), dim3(kThreads), 0, s,
+   hipLaunchKernelGGL(hr_dft_mfma<T>, grid_of_tiles((uint64_t) B * d0 * mC * iC * hC), dim3(kThreads), 0, s,
                       (const T *) src, dst, p2, N, B, d0, mC, iC, hC, shifts[4]);
    if ((e = hipGetLastError()) != hipSuccess) return e;
    if (!interleave) {
