@@ -188,7 +188,7 @@ struct MfmaArgs {
    const float2   *x;
    float2         *y;
    const uint16_t *frag;      // GEMM-1 B fragments of every part, then GEMM-2 A fragments of every part
-   const float2   *twN;       // twiddle rows: W_N^(r b) at r C + b (r < 16, b < C), then W_C^(b1 d2) at 16 C + 16 d2 + b1
+   const float2   *twN;       // twiddle rows: W_N^(r b) at r (C + 1) + b (r < 16, b < C), then W_C^(b1 d2) at 16 (C + 1) + 16 d2 + b1
    uint32_t        numChannels;
    int             mode;      // C2CMode
 };
@@ -224,13 +224,20 @@ constexpr int frag_halves()
    return 2 * SplitTraits<S>::parts * 64 * 8;
 }
 
-// Twiddle entries in LDS: the rows W_N^(r b) (r < 16, b < C = 16 R3), so that
-// each work item reads its 4 R3 twiddles at one base plus immediate offsets,
+// Twiddle entries in LDS: the rows W_N^(r b) (r < 16, b < C = 16 R3) with a
+// stride of C + 1 entries, so that each work item reads its 4 R3 twiddles at
+// one base plus immediate offsets and the 16 rows of a read fall in distinct
+// banks (a stride of C, a multiple of the bank count, put them all in one);
 // then W_C^(b1 d2) at 16 d2 + b1.
+template <int R3>
+constexpr int mfma_tw_row()
+{
+   return 16 * R3 + 1;
+}
 template <int R3>
 constexpr int mfma_tw_entries()
 {
-   return 16 * 16 * R3 + 16 * R3;
+   return 16 * mfma_tw_row<R3>() + 16 * R3;
 }
 
 // Outputs are staged in LDS per wavefront and stored 64 consecutive points
@@ -273,8 +280,9 @@ c2c_mfma_kernel(MfmaArgs a)
    const uint32_t wave = threadIdx.x / 64;
    const uint32_t q    = lane / 16, r = lane % 16;
    const bool     live = r < (uint32_t) R1;   // column c = r exists (the others are padding)
-   const float2  *twr  = tw + (live ? r * C : 0u) + 4u * q;   // W_N^(r (16 mt + 4q + i)) at 16 mt + i
-   const float2  *twc  = tw + 16 * C + 4u * q;                 // W_C^((4q + i) d2) at 16 d2 + i
+   constexpr int  kRow = mfma_tw_row<R3>();
+   const float2  *twr  = tw + (live ? r * kRow : 0u) + 4u * q;   // W_N^(r (16 mt + 4q + i)) at 16 mt + i
+   const float2  *twc  = tw + 16 * kRow + 4u * q;                // W_C^((4q + i) d2) at 16 d2 + i
 
    const float sxIn  = a.mode == C2C_INVERSE_PRESCALE ? 1.0f / (float) N : 1.0f;
    const float syIn  = a.mode == C2C_FORWARD ? sxIn : -sxIn;
@@ -537,7 +545,7 @@ hipError_t get_tables(int S, uint32_t n, int R1, int R3, const uint16_t **frag, 
    auto it = g_tables.find(TableKey{dev, S, n});
    if (it == g_tables.end()) {
       const std::vector<uint16_t> h = build_frags(S, R1, R3);
-      const uint32_t              C = 16u * (uint32_t) R3, entries = 16u * C + 16u * (uint32_t) R3;
+      const uint32_t              C = 16u * (uint32_t) R3, row = C + 1u, entries = 16u * row + 16u * (uint32_t) R3;
       std::vector<float2>         w(entries);
       const double                pi = 3.14159265358979323846;
       auto                        wn = [&](uint32_t m) {
@@ -545,9 +553,9 @@ hipError_t get_tables(int S, uint32_t n, int R1, int R3, const uint16_t **frag, 
          return make_float2((float) cos(ang), (float) sin(ang));
       };
       for (uint32_t r = 0; r < 16; ++r)
-         for (uint32_t b = 0; b < C; ++b) w[r * C + b] = wn(r * b);
+         for (uint32_t b = 0; b < C; ++b) w[r * row + b] = wn(r * b);
       for (uint32_t d2 = 0; d2 < (uint32_t) R3; ++d2)
-         for (uint32_t b1 = 0; b1 < 16; ++b1) w[16 * C + 16 * d2 + b1] = wn((uint32_t) R1 * b1 * d2);
+         for (uint32_t b1 = 0; b1 < 16; ++b1) w[16 * row + 16 * d2 + b1] = wn((uint32_t) R1 * b1 * d2);
       DeviceTables t;
       if ((e = hipMalloc((void **) &t.frag, h.size() * sizeof(uint16_t))) != hipSuccess) return e;
       if ((e = hipMalloc((void **) &t.twN, entries * sizeof(float2))) != hipSuccess) {
