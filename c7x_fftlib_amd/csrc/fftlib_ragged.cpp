@@ -86,6 +86,7 @@ struct FFTLIB_GPU_ragged_s {
    // persistent form
    int                form        = FFTLIB_RAGGED_GRAPH;   // the faster form on the ragged 10000 workload
    bool               formSet     = false;   // chosen by FFTLIB_GPU_raggedSetForm (the tuner keeps it)
+   bool                pAvailable = true;   // every length fits the persistent kernel (factors up to 4096)
    fixedk::ClassTask  *pTasks     = nullptr;
    fixedk::RaggedUnit *pUnits     = nullptr;
    uint32_t           *pCounters  = nullptr;
@@ -382,7 +383,7 @@ hipError_t build_persistent(FFTLIB_GPU_ragged_s *h)
       }
    }
    for (const auto &k : tasks)
-      if (k.n > kRaggedClassMaxN) return hipErrorInvalidValue;
+      if (k.n > kRaggedClassMaxN) return hipErrorInvalidValue;   // excluded when the plan was split
    const std::vector<ragged::QueueUnit> qu = ragged::build_units(q);
    if (qu.empty() || qu.size() > 0xFFFFFFFFull - 65536u) return hipErrorInvalidValue;
    std::vector<fixedk::RaggedUnit> units(qu.size());
@@ -456,11 +457,10 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
       // follow the graph form's
       g.pTwoPass = g.n > kRaggedClassMaxN;
       if (g.pTwoPass) {
-         if ((e = two_pass_plan(g.n, &g.pPlan)) != hipSuccess) {
-            release(h);
-            delete h;
-            return fail("ragged two-pass tables", e);
-         }
+         if (g.twoPass) g.pPlan = g.plan;
+         else if (two_pass_plan(g.n, &g.pPlan) != hipSuccess || g.pPlan.n1 > kRaggedClassMaxN ||
+                  g.pPlan.m2 > kRaggedClassMaxN)
+            h->pAvailable = false;   // the graph form still runs it in one pass
       }
       if ((e = hipMalloc((void **) &g.dOff, kv.second.size() * sizeof(uint64_t))) == hipSuccess)
          e = hipMemcpy(g.dOff, kv.second.data(), kv.second.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
@@ -492,12 +492,12 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
    // runs one form at a time), 256-byte aligned like the graph form's
    size_t pscr = 0;
    for (auto &g : h->groups)
-      if (g.pTwoPass) {
+      if (g.pTwoPass && h->pAvailable) {
          g.pScrOff = pscr / sizeof(float2);
          pscr += ((size_t) g.pPlan.n1 * g.pPlan.s2 * g.count * 8u + 255u) & ~(size_t) 255u;
       }
    scr = std::max(scr, pscr);
-   if ((e = build_persistent(h)) != hipSuccess) {
+   if (h->pAvailable && (e = build_persistent(h)) != hipSuccess) {
       release(h);
       delete h;
       return fail("ragged persistent tables", e);
@@ -505,7 +505,8 @@ FFTLIB_STATUS FFTLIB_GPU_raggedCreate(FFTLIB_GPU_raggedHandle *handle, const uin
    {
       uint32_t g = 0;
       if (tuned(kPersistentGridKey, h->pNumItems, &g)) h->pGrid = g;
-      if (tuned(kFormKey, form_count(h), &g) && (g == FFTLIB_RAGGED_GRAPH || g == FFTLIB_RAGGED_PERSISTENT))
+      if (tuned(kFormKey, form_count(h), &g) &&
+          (g == FFTLIB_RAGGED_GRAPH || (g == FFTLIB_RAGGED_PERSISTENT && h->pAvailable)))
          h->form = (int) g;
    }
    if (scr) {
@@ -647,7 +648,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    };
    // persistent form: grid candidates (0 = the resident workgroups)
    float persistUs = 1e30f;
-   if (e == hipSuccess) {
+   if (e == hipSuccess && handle->pAvailable) {
       const uint32_t res    = ragged_class_resident();
       const uint32_t cand[] = {0u, res * 3u / 4u, res / 2u, res / 4u};
       uint32_t       bestG  = 0;
@@ -687,6 +688,11 @@ FFTLIB_STATUS FFTLIB_GPU_raggedSetForm(FFTLIB_GPU_raggedHandle handle, int32_t f
    if (handle == nullptr) return FFTLIB_ERR_NULL_POINTER;
    if (form != FFTLIB_RAGGED_GRAPH && form != FFTLIB_RAGGED_PERSISTENT) return FFTLIB_ERR_INVALID_VALUE;
    std::lock_guard<std::mutex> lock(handle->mu);
+   if (form == FFTLIB_RAGGED_PERSISTENT && !handle->pAvailable) {
+      set_last_error_text("ragged plan: a length's two-pass split has a factor above 4096, which the persistent "
+                          "kernel does not hold; the graph form runs it");
+      return FFTLIB_ERR_NOT_IMPLEMENTED;
+   }
    if (handle->form != form) {   // the captured graph (if any) is of the other form
       if (handle->exec) (void) hipGraphExecDestroy(handle->exec), handle->exec = nullptr;
       if (handle->graph) (void) hipGraphDestroy(handle->graph), handle->graph = nullptr;
