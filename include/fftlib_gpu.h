@@ -97,7 +97,9 @@ int32_t       FFTLIB_GPU_getC2CVariant(void);
  *     direction runs every transform from a work queue; each transform
  *     longer than 4096 points is one unit of the queue, run by one workgroup
  *     as a column pass into its own scratch region and a row pass after a
- *     workgroup barrier (no unit waits for another).
+ *     workgroup barrier (no unit waits for another). Setting it returns
+ *     FFTLIB_ERR_NOT_IMPLEMENTED for a plan with a length whose two-pass
+ *     split needs a factor above 4096 (none up to 65536 points).
  *   FFTLIB_RAGGED_GRAPH (default): one multi-length kernel launch per octave
  *     of lengths and pass, forked over several streams and joined per pass.
  * The tuning cache also holds the persistent launch's workgroup count (key
