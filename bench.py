@@ -44,8 +44,8 @@ sys.path.insert(0, ROOT)
 
 RAGGED_FORMS = {"graph": 0, "persistent": 1}   # FFTLIB_RAGGED_GRAPH, FFTLIB_RAGGED_PERSISTENT
 RAGGED_KERNELS = {
-    "persistent": "ragged_class_kernel (one persistent launch per direction: lengths <= 4096 whole, longer "
-                  "lengths as column and row groups that wait inside the launch)",
+    "persistent": "ragged_class_kernel (one persistent launch per direction: lengths <= 4096 whole, each "
+                  "longer transform a column then a row pass in one workgroup)",
     "graph": "ragged graph (class kernels: lengths <= 16384 in one pass, longer lengths in a column and a row pass)",
 }
 METRIC = "Batched C2C FFT GFLOP/s & %HBM peak; split-BF16 MFMA vs FP32 at ≤2e-6 err"
