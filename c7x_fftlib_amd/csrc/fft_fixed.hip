@@ -253,8 +253,11 @@ bool two_pass_split_above(uint32_t n, uint32_t above, uint32_t *n1, uint32_t *m2
             for (int r = 0; r < k; ++r) d *= 5u;
             const uint64_t e = n / d;
             if (d < 16u || d > 16384u || e < 16u || e > 16384u) continue;
-            const int score = ((d <= 256u && e <= 256u) ? 8 : 0) + (d % 16u == 0u ? 4 : d % 8u == 0u ? 2 : 0) +
-                              (e % 16u == 0u ? 2 : e % 8u == 0u ? 1 : 0);
+            // N1 = e a multiple of 16 first (row-pass outputs on whole
+            // lines, see rowAlign), then M2 = d (column reads and the
+            // intermediate rows without idle columns)
+            const int score = ((d <= 256u && e <= 256u) ? 8 : 0) + (e % 16u == 0u ? 4 : e % 8u == 0u ? 2 : 0) +
+                              (d % 16u == 0u ? 2 : d % 8u == 0u ? 1 : 0);
             const double bal = fabs(log((double) d) - log((double) e));
             if (score > bestScore || (score == bestScore && bal < bestBal)) {
                bestScore = score, bestBal = bal, best = (uint32_t) d;
@@ -344,6 +347,13 @@ ClassTask two_pass_task(const TwoPassPlan &p, int layout, uint32_t count, const 
    k.colTw       = (const float2 *) p.colTw;
    k.colTwHi     = (const float2 *) p.colTwHi;
    k.colBits     = p.colBits;
+   // rows rotated so that the row groups' output runs fill 128-byte lines
+   // (A = 16 points), else 64-byte halves (A = 8): needs N1 a multiple of A
+   // (every output row k2 starts at the same offset within a line, and each
+   // A-row block of a transform's rows lies in one row group) and row groups
+   // of a multiple of A rows
+   const uint32_t rowFpb = fixed_fpb(p.m2);
+   k.rowAlign    = (p.n1 % 16u == 0u && rowFpb % 16u == 0u) ? 16u : (p.n1 % 8u == 0u && rowFpb % 8u == 0u) ? 8u : 0u;
    return k;
 }
 

@@ -241,9 +241,34 @@ struct FixedArgs {
    const float2 *colTwHi;     // ... and W_fullN^(h 2^colBits): W^e = colTwHi[e >> colBits] colTw[e & mask]
    uint32_t      colBits;
    uint32_t      scrStride;   // two-pass: points per row of the intermediate copy (span of the columns rounded up to 16)
+   // Two-pass with rowAlign = A (16 or 8 points, else 0): the rows of
+   // transform f are rotated in the intermediate copy (row r holds k1 =
+   // (r + rot_f) mod N1, rot_f from the address of its output in yOut, see
+   // row_rotation) so that the row groups' output runs start on 8 A-byte
+   // boundaries.
+   uint32_t      rowAlign;
+   const float2 *yOut;        // the final output (also in the column pass, whose y is the intermediate copy)
 };
 
 enum : uint32_t { LAYOUT_PLAIN = 0u, LAYOUT_COLUMNS = 1u, LAYOUT_ROWS = 2u };
+
+// Row rotation of a two-pass transform of N1 = n1 rows whose output starts
+// at `out`, for A = rowAlign (N1 and the row groups' FPB multiples of A): 0
+// when out is on an 8 A-byte boundary, else N1 - A + s with s the points
+// from out to the next one. Each block of A rows A c .. A c + A - 1 (within
+// one row group) then holds k1 from A c - A + s on, so that its outputs
+// k1 + N1 k2 fill whole 8 A-byte pieces (block 0's rows k1 >= N1 - A + s at
+// k2 and k1 < s at k2 + 1 share pieces too). A = 16 fills 128-byte lines: a
+// line written in parts by several workgroups, on different dies, is
+// written back in parts, and the partial writes cost HBM read-modify-write
+// cycles (2^17 x 65536-point transforms: row pass 400 us with the output on
+// a line, 520 us 64 bytes off it, 620 us 8 bytes off it).
+__device__ __forceinline__ uint32_t row_rotation(const FixedArgs &a, const float2 *out, uint32_t n1)
+{
+   if (!a.rowAlign) return 0u;
+   const uint32_t s = (uint32_t) ((0u - (uintptr_t) out) >> 3) & (a.rowAlign - 1u);
+   return s ? n1 - a.rowAlign + s : 0u;
+}
 
 // First complex element of channel ch in the input and in the output.
 __device__ __forceinline__ size_t in_base(const FixedArgs &a, uint32_t ch)
@@ -1267,6 +1292,7 @@ struct ClassTask {
    const float2   *colTwHi;
    uint32_t        colBits;
    uint32_t        scrStride;    // two-pass: see FixedArgs
+   uint32_t        rowAlign;     // two-pass: see FixedArgs
 };
 
 // Work unit of the persistent ragged launch: groups [g0, g1) of task, then,
@@ -1349,7 +1375,9 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
    __syncthreads();
    fixed_passes_from<N, 0>(smem + (threadIdx.x / T) * STRIDE, threadIdx.x % T, a.tw);
    if (live) {
-      float2 *out = a.y + (size_t) f * ((size_t) N * a.scrStride) + p;
+      const size_t   obase = a.inOff ? (size_t) a.inOff[f] : (size_t) f * a.fullN;   // the transform's place in yOut
+      const uint32_t rot   = row_rotation(a, a.yOut + obase, (uint32_t) N);
+      float2        *out   = a.y + (size_t) f * ((size_t) N * a.scrStride) + p;
 #pragma unroll 1
       for (int c0 = 0; c0 < IT; c0 += CH) {
          float2 v[CH], w[CH];
@@ -1372,9 +1400,10 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
 #pragma unroll
          for (int c = 0; c < CH; ++c) {
             const uint32_t k = i0 + (uint32_t) ((c0 + c) * T);
+            const uint32_t row = k >= rot ? k - rot : k + (uint32_t) N - rot;   // k1 = (row + rot) mod N
             if ((N % T == 0 && IT % CH == 0) || (c0 + c < IT && k < (uint32_t) N))
-               out[(size_t) a.scrStride * k] = make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
-                                                      v[c].x * w[c].y + v[c].y * w[c].x);
+               out[(size_t) a.scrStride * row] = make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
+                                                        v[c].x * w[c].y + v[c].y * w[c].x);
          }
       }
    }
@@ -1399,9 +1428,11 @@ __device__ __forceinline__ void rows_store(const float2 *smem, uint32_t first, c
    const uint32_t      i0     = threadIdx.x / FPB;
    const uint32_t      ch     = first + j;
    if (ch >= a.numChannels) return;
-   const uint32_t f   = ch / a.span;
-   const uint32_t r   = ch - f * a.span;
-   float2        *out = a.y + (a.outOff ? (size_t) a.outOff[f] : (size_t) f * a.fullN) + r;
+   const uint32_t f    = ch / a.span;
+   const uint32_t r    = ch - f * a.span;
+   float2        *outf = a.y + (a.outOff ? (size_t) a.outOff[f] : (size_t) f * a.fullN);
+   const uint32_t rot  = row_rotation(a, outf, a.span);
+   float2        *out  = outf + (r + rot < a.span ? r + rot : r + rot - a.span);   // row r holds k1 = (r + rot) mod N1
    const float2  *src = smem + j * STRIDE;
    const float    sy  = a.mode == C2C_INVERSE_PRESCALE ? -1.0f : 1.0f;
    const bool     pdiv = a.mode == C2C_INVERSE_POSTDIV;
@@ -1498,6 +1529,8 @@ __device__ __forceinline__ FixedArgs class_fixed_args(const C &c, const K &k)
    a.colTwHi     = k.colTwHi;
    a.colBits     = k.colBits;
    a.scrStride   = k.scrStride;
+   a.rowAlign    = k.rowAlign;
+   a.yOut        = c.y;
    return a;
 }
 

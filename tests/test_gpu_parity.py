@@ -346,6 +346,43 @@ def test_c2c_large_two_pass_longest():
     assert G.rel_l2(y, oracle.c2c(x, n, 2)) <= REL_TOL
 
 
+@pytest.mark.parametrize("n", [20480, 20000, 48600, 16875, 65536])
+def test_c2c_two_pass_output_offsets(n):
+    """Two-pass transforms with input and output 8, 64 and 136 bytes off a
+    128-byte line: the rows of the intermediate copy are rotated by the output
+    address where N1 is a multiple of 16 or 8 (N1 x M2: 20480 = 128 x 160,
+    20000 = 160 x 125, 65536 = 256 x 256; 48600 = 216 x 225 to 64-byte
+    pieces; 16875 = 135 x 125 keeps them in order), which moves points but
+    not their arithmetic, so every offset gives the aligned call's bits."""
+    import torch
+    b = 5
+    x = cplx(rng(n + 11), b, n, 1.0)
+    L = G.lib()
+    G._set_style("gpu")
+    xin = np.ascontiguousarray(x.astype(np.complex64)).view(np.float32).reshape(-1)
+    w = torch.from_numpy(G.tw_c2c(n)).cuda()
+    name = "FFTLIB_fft1dBatched_i32fc_c32fc_o32fc"
+    bx, bw = G._bp(G.F.FFTLIB_FLOAT32, 2 * n * b), G._bp(G.F.FFTLIB_FLOAT32, 2 * n)
+    outs = {}
+    for off in (0, 8, 64, 136):
+        xt = torch.zeros(xin.size + 64, dtype=torch.float32, device="cuda:0")
+        yt = torch.full((xin.size + 64,), float("nan"), dtype=torch.float32, device="cuda:0")
+        k = off // 4
+        xt[k:k + xin.size] = torch.from_numpy(xin).cuda()
+        xp, yp = xt.data_ptr() + off, yt.data_ptr() + off
+        print(f"[parity] two-pass n={n} b={b} offset {off} B", flush=True)
+        pb = G.pblock(name)
+        assert getattr(L, name + "_init")(xp, bx, w.data_ptr(), bw, yp, bx, n, b, pb) == 0
+        assert getattr(L, name + "_kernel")(xp, bx, w.data_ptr(), bw, yp, bx, n, b, pb) == 0, G.F.last_error()
+        torch.cuda.synchronize()
+        y = yt.cpu().numpy()
+        assert np.isnan(y[:k]).all() and np.isnan(y[k + xin.size:]).all(), "wrote outside the output"
+        outs[off] = y[k:k + xin.size].view(np.complex64).reshape(b, n)
+    assert G.rel_l2(outs[0], oracle.c2c(x, n, b)) <= REL_TOL
+    for off in (8, 64, 136):
+        assert np.array_equal(outs[off].view(np.uint32), outs[0].view(np.uint32)), off
+
+
 @pytest.mark.parametrize("kind", ["fft", "ifft"])
 @pytest.mark.parametrize("n", [20000, 65536])
 def test_c2c_large_generic(n, kind, generic_kernels):
