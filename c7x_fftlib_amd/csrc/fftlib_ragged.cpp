@@ -122,7 +122,8 @@ bool smooth235(uint32_t n)
 // Key: the class-kernel launch (kClassKeyBase + 100 * phase + octave index,
 // plus 1000 * (1 + octave of the column tasks) for row launches) and a
 // power-of-two bucket of its item count; value: the launch's number of
-// workgroups (0 = the class kernels' default) and its time.
+// workgroups (0 = the class kernels' default) and the time of a forward
+// direction of the plan it was tuned in, with that grid.
 constexpr uint32_t kClassKeyBase = 100000u;
 uint32_t class_key(const FFTLIB_GPU_ragged_s::Class &c)
 {
@@ -601,35 +602,9 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    if ((e = hipEventCreate(&t1)) != hipSuccess) return fail("hipEventCreate", e);
    hipStream_t s = handle->capture;
    if ((e = order_after_caller(handle)) != hipSuccess) return fail("ragged stream ordering", e);
-   // phase-0 launches precede phase-1 ones, so the row tasks read column
-   // results of the same input
-   for (auto &c : handle->classes) {
-      if (e != hipSuccess) break;
-      // candidates: the default (0: up to 8 x the resident workgroups), then
-      // multiples of the CU count, each at most the launch's item count
-      const uint32_t u      = (uint32_t) cus;
-      const uint32_t cand[] = {0u, 32u * u, 16u * u, 8u * u, 4u * u, 2u * u, u, u / 2u};
-      float          best   = 1e30f;
-      uint32_t       bestG  = 0;
-      for (uint32_t cap : cand) {
-         if ((e = launch_class(*handle, c, pX, pY, C2C_FORWARD, cap, s)) != hipSuccess) break;   // warm
-         float ms = 0.0f;
-         if ((e = hipEventRecord(t0, s)) != hipSuccess) break;
-         for (int r = 0; r < 3 && e == hipSuccess; ++r) e = launch_class(*handle, c, pX, pY, C2C_FORWARD, cap, s);
-         if (e != hipSuccess) break;
-         if ((e = hipEventRecord(t1, s)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess ||
-             (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess)
-            break;
-         if (ms < best) best = ms, bestG = cap;
-      }
-      if (e != hipSuccess) break;
-      c.grid = bestG;
-      std::lock_guard<std::mutex> tl(g_tuneMu);
-      g_tune[TuneKey{class_key(c), bucket_of(c.numItems)}] = TuneVal{bestG, best * 1000.0f / 3.0f};
-   }
-   // one forward direction of a form, eager, 3 timed after 1 warm (us)
+   // one forward direction of a form, eager, `reps` timed after 1 warm (us)
    const int savedForm  = handle->form;
-   auto      time_form = [&](int form, uint32_t grid, float *us) {
+   auto      time_form = [&](int form, uint32_t grid, int reps, float *us) {
       handle->form = form;
       const uint32_t savedGrid = handle->pGrid;
       handle->pGrid            = grid;
@@ -637,15 +612,46 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
       hipError_t      r = enqueue(handle, &fwd, 1, s);
       float           ms = 0.0f;
       if (r == hipSuccess) r = hipEventRecord(t0, s);
-      for (int k = 0; k < 3 && r == hipSuccess; ++k) r = enqueue(handle, &fwd, 1, s);
+      for (int k = 0; k < reps && r == hipSuccess; ++k) r = enqueue(handle, &fwd, 1, s);
       if (r == hipSuccess) r = hipEventRecord(t1, s);
       if (r == hipSuccess) r = hipEventSynchronize(t1);
       if (r == hipSuccess) r = hipEventElapsedTime(&ms, t0, t1);
       handle->pGrid = savedGrid;
       handle->form  = savedForm;
-      *us           = ms * 1000.0f / 3.0f;
+      *us           = ms * 1000.0f / (float) reps;
       return r;
    };
+   // graph form: the grid of each class launch by one sweep of coordinate
+   // descent on the time of a whole direction (the launches of a phase run
+   // side by side, and the grid with which a launch runs fastest alone made
+   // the ragged 10000 direction 3% slower than the defaults did), from the
+   // grids the plan holds, launches in plan order; a candidate replaces the
+   // current grid only when the direction is at least 1% faster with it
+   float graphUs = 0.0f;
+   e             = time_form(FFTLIB_RAGGED_GRAPH, 0u, 5, &graphUs);
+   for (auto &c : handle->classes) {
+      if (e != hipSuccess) break;
+      // candidates: the default (0: up to 8 x the resident workgroups), then
+      // multiples of the CU count (the launch clamps them to its item count)
+      const uint32_t u      = (uint32_t) cus;
+      const uint32_t cand[] = {0u, 32u * u, 16u * u, 8u * u, 4u * u, 2u * u, u, u / 2u};
+      const uint32_t cur    = c.grid;
+      uint32_t       bestG  = cur;
+      float          best   = graphUs;
+      for (uint32_t cap : cand) {
+         if (cap == cur) continue;
+         c.grid   = cap;
+         float us = 0.0f;
+         if ((e = time_form(FFTLIB_RAGGED_GRAPH, 0u, 5, &us)) != hipSuccess) break;
+         if (us < 0.99f * best) best = us, bestG = cap;
+      }
+      c.grid = bestG;
+      if (e != hipSuccess) break;
+      graphUs = best;
+      std::lock_guard<std::mutex> tl(g_tuneMu);
+      g_tune[TuneKey{class_key(c), bucket_of(c.numItems)}] = TuneVal{bestG, best};
+   }
+   if (e == hipSuccess) e = time_form(FFTLIB_RAGGED_GRAPH, 0u, 5, &graphUs);
    // persistent form: grid candidates (0 = the resident workgroups)
    float persistUs = 1e30f;
    if (e == hipSuccess && handle->pAvailable) {
@@ -654,7 +660,7 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
       uint32_t       bestG  = 0;
       for (uint32_t g : cand) {
          float us = 0.0f;
-         if ((e = time_form(FFTLIB_RAGGED_PERSISTENT, g, &us)) != hipSuccess) break;
+         if ((e = time_form(FFTLIB_RAGGED_PERSISTENT, g, 3, &us)) != hipSuccess) break;
          if (us < persistUs) persistUs = us, bestG = g;
       }
       if (e == hipSuccess) {
@@ -665,14 +671,10 @@ FFTLIB_STATUS FFTLIB_GPU_raggedTune(FFTLIB_GPU_raggedHandle handle, const FFTLIB
    }
    // the form: the faster one, unless the caller chose it
    if (e == hipSuccess) {
-      float graphUs = 0.0f;
-      if ((e = time_form(FFTLIB_RAGGED_GRAPH, 0u, &graphUs)) == hipSuccess) {
-         const int form = graphUs < persistUs ? FFTLIB_RAGGED_GRAPH : FFTLIB_RAGGED_PERSISTENT;
-         if (!handle->formSet) handle->form = form;
-         std::lock_guard<std::mutex> tl(g_tuneMu);
-         g_tune[TuneKey{kFormKey, bucket_of(form_count(handle))}] =
-            TuneVal{(uint32_t) form, std::min(graphUs, persistUs)};
-      }
+      const int form = graphUs < persistUs ? FFTLIB_RAGGED_GRAPH : FFTLIB_RAGGED_PERSISTENT;
+      if (!handle->formSet) handle->form = form;
+      std::lock_guard<std::mutex> tl(g_tuneMu);
+      g_tune[TuneKey{kFormKey, bucket_of(form_count(handle))}] = TuneVal{(uint32_t) form, std::min(graphUs, persistUs)};
    }
    (void) hipEventDestroy(t0);
    (void) hipEventDestroy(t1);

@@ -80,13 +80,16 @@ int32_t       FFTLIB_GPU_getC2CVariant(void);
  * serialised; calls must not overlap with other calls that use the same
  * handle's buffers.
  *
- * FFTLIB_GPU_raggedTune times the candidate grid sizes (workgroup counts)
- * of every multi-length kernel the plan launches (one per octave of lengths)
- * with the caller's buffers (pY is overwritten) and keeps the fastest in a
- * process-wide cache that later plans with the same kernels and work sizes
- * use; FFTLIB_GPU_tuneSave / FFTLIB_GPU_tuneLoad write and read that cache as
+ * FFTLIB_GPU_raggedTune chooses the grid size (workgroup count) of every
+ * multi-length kernel the plan launches (one per octave of lengths) by
+ * timing whole forward directions with the caller's buffers (pY is
+ * overwritten): one launch at a time, each candidate with the others' grids
+ * fixed, a candidate kept when the direction is at least 1% faster; the
+ * choices go to a process-wide cache that later plans with the same kernels
+ * and work sizes use; FFTLIB_GPU_tuneSave / FFTLIB_GPU_tuneLoad write and read that cache as
  * text ("fftlib-amd-tune 1", then one "key item-count-bucket grid
- * microseconds" line per entry, grid 0 = the kernel's default,
+ * microseconds" line per entry, grid 0 = the kernel's default, microseconds
+ * = the tuned direction's time,
  * key = 100000 + 100 * phase + octave index;
  * phase 0 = lengths up to 16384 and the column pass of longer lengths,
  * phase 1 = the row pass of longer lengths).
