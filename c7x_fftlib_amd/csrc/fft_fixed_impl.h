@@ -1363,7 +1363,7 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
          const uint32_t i = min(i0 + (uint32_t) ((c0 + c) * T), (uint32_t) N - 1u);
-         v[c]             = in[(size_t) a.span * i];
+         v[c]             = load_stream(in + (size_t) a.span * i);
       }
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
@@ -1402,8 +1402,8 @@ __device__ __forceinline__ void column_group(const FixedArgs &a, uint32_t g, flo
             const uint32_t k = i0 + (uint32_t) ((c0 + c) * T);
             const uint32_t row = k >= rot ? k - rot : k + (uint32_t) N - rot;   // k1 = (row + rot) mod N
             if ((N % T == 0 && IT % CH == 0) || (c0 + c < IT && k < (uint32_t) N))
-               out[(size_t) a.scrStride * row] = make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
-                                                        v[c].x * w[c].y + v[c].y * w[c].x);
+               store_stream(out + (size_t) a.scrStride * row, make_float2(v[c].x * w[c].x - v[c].y * w[c].y,
+                                                                         v[c].x * w[c].y + v[c].y * w[c].x));
          }
       }
    }
@@ -1446,7 +1446,7 @@ __device__ __forceinline__ void rows_store(const float2 *smem, uint32_t first, c
       for (int c = 0; c < CH; ++c) {
          const uint32_t k = i0 + (uint32_t) ((c0 + c) * T);
          if (!(N % T == 0 && IT % CH == 0) && (c0 + c >= IT || k >= (uint32_t) N)) break;
-         out[(size_t) a.span * k] = pdiv ? out_postdiv(v[c], nf) : make_float2(v[c].x, sy * v[c].y);
+         store_stream(out + (size_t) a.span * k, pdiv ? out_postdiv(v[c], nf) : make_float2(v[c].x, sy * v[c].y));
       }
    }
 }
